@@ -1,0 +1,145 @@
+/*
+ * swmi.h -- C ABI of the MI355X-native Swendsen-Wang engine for the 2D Ising model.
+ *
+ * Drop-in for the Ising2D --algorithm=swendsen_wang --rng=lcg32 path of flwende/computational_physics.
+ * Plain C: opaque handles, plain pointers and sizes, int status codes (SWMI_OK = 0), no exceptions and
+ * no torch types. The last error message of the calling thread is swmi_last_error().
+ *
+ * Conventions shared with the reference:
+ *   spins   uint8 in {0,1} (1 = up), row-major, site index y*n0 + x          (ising/include/lattice.hpp:24)
+ *   labels  uint32, the minimum row-major site index of the cluster          (ising/src/swendsen_wang_cpu.cpp:124-181)
+ *   flip    spin ^= label & 1 (flip_rule SWMI_FLIP_PARITY)                   (ising/src/swendsen_wang_cpu.cpp:183-205)
+ *   energy  E = sum s(s_right + s_down), M = sum s with s = 2*spin - 1;
+ *           per-spin values are -E/N and M/N                                 (ising/src/lattice.cpp:29-67)
+ *   bonds   p_add = 1.0f - (float)expf(-2.0f / T), bond iff spins equal and r < p_add,
+ *           evaluated as u < threshold on the uint32 draw u                  (ising/src/swendsen_wang.cpp:100)
+ * lcg32 layout (new, the engine's own): group g = sites [64g, 64g+64) is one LCG32_State<64>
+ * (random/src/lcg32.cpp:31-84) seeded with seed*0x9E3779B1 + g + 1; each sweep draws twice per site,
+ * the first draw decides the bond to (x+1, y), the second the bond to (x, y+1).
+ */
+#ifndef SWMI_H
+#define SWMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+    SWMI_OK = 0,
+    SWMI_ERR_INVALID = 1,     /* bad argument (extent, temperature, index, size) */
+    SWMI_ERR_HIP = 2,         /* HIP runtime error */
+    SWMI_ERR_NO_DEVICE = 3,   /* no usable gfx950 device */
+    SWMI_ERR_MEMORY = 4,      /* device memory or budget exhausted */
+    SWMI_ERR_STATE = 5        /* call not valid in the current state */
+};
+
+enum
+{
+    SWMI_FLIP_PARITY = 0 /* flip the cluster iff its canonical root index is odd (reference rule) */
+};
+
+typedef struct swmi_engine swmi_engine;
+
+typedef struct
+{
+    uint32_t n0;          /* extent in x (fast index); even; a multiple of 64 or a divisor of 64 */
+    uint32_t n1;          /* extent in y; even; n0*n1 a multiple of 64 and < 2^31 */
+    float temperature;    /* T > 0 */
+    uint32_t seed;        /* lcg32 group seeds: seed*0x9E3779B1 + g + 1 (seed 0: g + 1) */
+    uint32_t spin_seed;   /* hot start: srand48(spin_seed); spin = drand48() < 0.5 ? 0 : 1 (1: reference) */
+    uint32_t tile_rows;   /* tile height in 64-site wave rows (0: default); results do not depend on it */
+    uint32_t flip_rule;   /* SWMI_FLIP_PARITY */
+    uint32_t max_samples; /* capacity of the on-device (E, M) time series (0: no series, moments only) */
+} swmi_lattice_config;
+
+typedef struct
+{
+    char name[64];
+    char arch[32];
+    uint32_t compute_units;
+    uint32_t wavefront_size;
+    uint64_t total_memory;
+    uint32_t l2_bytes;
+    uint32_t clock_khz;
+} swmi_device_info;
+
+/* ---- errors and device ---- */
+const char* swmi_last_error(void);
+/* Replaces AMD_GPU / HipGetDeviceProperty (utility/device/device_amd_gpu.hpp:15-87,
+ * utility/hip/hip.cpp:26-101): device 0 properties. */
+int swmi_get_device_info(swmi_device_info* out);
+/* Library version string. */
+const char* swmi_version(void);
+
+/* ---- engine lifecycle: one engine owns one or more lattices (a batch); calls on one engine are
+ * serialised on its own HIP stream. Replaces the Lattice<2> constructor (ising/src/lattice.cpp:15-27),
+ * Lattice::InitializeGpuSpins (lattice.cpp:81-97) and SwendsenWang_2D::Initialize
+ * (ising/src/swendsen_wang.cpp:41-91, swendsen_wang_amd_gpu.cpp:22-53). ---- */
+int swmi_create(const swmi_lattice_config* configs, uint32_t n_lattices, swmi_engine** out);
+int swmi_destroy(swmi_engine* engine);
+int swmi_num_lattices(const swmi_engine* engine, uint32_t* out);
+int swmi_synchronize(swmi_engine* engine);
+
+/* ---- state transfer (host buffers of n0*n1 bytes) ---- */
+int swmi_upload_spins(swmi_engine* engine, uint32_t lattice, const uint8_t* spins);
+int swmi_download_spins(swmi_engine* engine, uint32_t lattice, uint8_t* spins);
+
+/* ---- temperature: p_add and its integer threshold are derived on the host
+ * (ising/src/swendsen_wang.cpp:100). ---- */
+int swmi_set_temperature(swmi_engine* engine, uint32_t lattice, float temperature);
+int swmi_get_threshold(const swmi_engine* engine, uint32_t lattice, float* p_add, uint64_t* threshold);
+
+/* ---- sweeps: every lattice of the engine advances n_sweeps Swendsen-Wang sweeps. Replaces
+ * LatticeMonteCarloAlgorithm::Update (ising/include/lattice_mc.hpp:18) as implemented by
+ * SwendsenWang_2D::Update (ising/src/swendsen_wang.cpp:94-112). Asynchronous on the engine stream. ---- */
+int swmi_sweep(swmi_engine* engine, uint32_t n_sweeps);
+
+/* ---- observables. swmi_sample appends one sample (E, M of the current configuration) per lattice to
+ * its on-device series and FP64 moments; swmi_run runs n_sweeps sweeps with a sample after every
+ * n_sep-th sweep (the measurement loop of ising/src/ising_2d.cpp:126-148, all on the device). ---- */
+int swmi_sample(swmi_engine* engine);
+int swmi_run(swmi_engine* engine, uint32_t n_sweeps, uint32_t n_sep);
+/* Sums of the current configuration. Replaces Lattice<2>::GetEnergyAndMagnetization
+ * (ising/include/lattice.hpp:66-69, ising/src/lattice.cpp:29-79) without the full-lattice copy. */
+int swmi_observables(swmi_engine* engine, uint32_t lattice, int64_t* energy, int64_t* magnetization);
+/* moments[0..6] = {samples, sum e, sum e^2, sum |m|, sum m^2, sum m^4} with e = -E/N, m = M/N (FP64). */
+int swmi_get_moments(swmi_engine* engine, uint32_t lattice, double* moments);
+/* Copies up to max samples of the (E, M) series; *count receives the number stored. */
+int swmi_get_series(swmi_engine* engine, uint32_t lattice, int64_t* energy, int64_t* magnetization,
+                    uint32_t max, uint32_t* count);
+int swmi_reset_measurements(swmi_engine* engine);
+
+/* ---- inspection hooks (parity tests) ---- */
+/* Bond bytes per site (bit 0: bond to (x+1,y), bit 1: bond to (x,y+1)) drawn by the last sweep;
+ * recorded only while enabled. */
+int swmi_record_bonds(swmi_engine* engine, int enable);
+int swmi_download_bonds(swmi_engine* engine, uint32_t lattice, uint8_t* bonds);
+/* Label the given bond field (same byte format) without touching spins or lcg32 state, and leave the
+ * canonical labels for swmi_download_labels. */
+int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds);
+/* Canonical labels of the last labelling (after a sweep: the clusters that sweep flipped). */
+int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels);
+/* Per-site lcg32 states (n_groups*64), selector bit planes (3*n_groups), rotations (n_groups) and the
+ * iteration count of the lattice. */
+int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uint64_t* masks, uint32_t* rot,
+                    uint32_t* iteration);
+
+/* Invariant violations seen by the kernels (bit 0: a label chase left [0, n)); 0 in a correct run. */
+int swmi_get_error_flags(swmi_engine* engine, uint32_t lattice, uint32_t* flags);
+
+/* ---- measurement support ---- */
+/* Average device time per kernel over n_sweeps sweeps, measured with HIP events on the engine stream:
+ * ms[0] bond+label, ms[1] merge, ms[2] flip. */
+int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms);
+/* Device-to-device copy of `bytes` bytes (calibration kernel for HBM byte counters). */
+int swmi_copy_calibration(size_t bytes, uint32_t repeats, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
