@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Swendsen-Wang engine (one JSON line on stdout).
+
+Headline workload: plain Swendsen-Wang on a 4096 x 4096 periodic 2D Ising lattice at
+T = 2.2691853 (float), hot start (srand48(1)), per-site lcg32 seeded from seed 0, everything resident in
+HBM. A "step" is one sweep (bond activation + labelling + merge + flip of all 16.8M spins); `value` is
+spin-updates/s = steps * N / seconds between two device synchronisations.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 4096] [--temperature T] [--no-cpu-baseline]
+
+Roofline: the dominant kernel (bond activation + tile labelling) is HBM-bound; its algorithmic bytes
+are 13 per site per launch (spin read 1, lcg32 state read+write 8, label write 4). The whole sweep moves
+19 bytes per site (plus the flip: label read 4, spin read+write 2).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E
+BOND_LABEL_BYTES_PER_SITE = 13  # spin 1 + lcg32 r/w 8 + label w 4
+SWEEP_BYTES_PER_SITE = 19       # + label r 4 + spin r/w 2
+T_C = 2.2691853
+
+
+def cpu_baseline(n: int, temperature: float) -> dict | None:
+    """The reference CPU path (oracle/_ref/ref_sw, built from the reference sources) on the host cores,
+    on a bounded sample of the same workload; falls back to the single-thread C restatement."""
+    cores = min(16, len(os.sched_getaffinity(0)))
+    warmup, sweeps = 10, 100
+    ref = ROOT / "oracle" / "_ref" / "ref_sw"
+    if ref.exists():
+        env = dict(os.environ, NUM_THREADS=str(cores))
+        try:
+            out = subprocess.run([str(ref), "run", str(n), str(n), str(temperature), str(warmup), str(sweeps)], env=env,
+                                 capture_output=True, text=True, timeout=300, check=True).stdout
+            ns = float(re.search(r"Update time per spin: ([0-9.eE+-]+) ns", out).group(1))
+            return {"value": 1e9 / ns, "unit": "spin-updates/s", "cores": cores, "kind": "reference",
+                    "sample": f"{n}x{n} T={temperature}, {warmup} warmup + {sweeps} sweeps with observables every 10 "
+                              f"sweeps (ising_2d.cpp protocol), NUM_THREADS={cores}, lcg32 W=16 (AVX-512 build)"}
+        except (subprocess.SubprocessError, AttributeError, ValueError) as exc:
+            sys.stderr.write(f"reference CPU baseline failed: {exc}\n")
+    try:
+        import numpy as np
+        from oracle import oracle
+        r = oracle.RefCpu(n, n, 16, cores)
+        s = oracle.hot_start(n * n, 1)
+        k = 3
+        t0 = time.perf_counter()
+        for _ in range(k):
+            r.sweep(s, temperature)
+        dt = time.perf_counter() - t0
+        return {"value": k * n * n / dt, "unit": "spin-updates/s", "cores": 1, "kind": "port",
+                "sample": f"{n}x{n} T={temperature}, {k} sweeps of the single-threaded C restatement"}
+    except Exception as exc:  # the baseline is informative only
+        sys.stderr.write(f"CPU baseline unavailable: {exc}\n")
+        return None
+
+
+def load_traffic(n: int) -> float | None:
+    """HBM bytes per k_bond_label launch from the committed rocprofv3 PMC summary (scaled FETCH+WRITE)."""
+    f = ROOT / "profiles" / f"traffic_bond_label_{n}.json"
+    if f.exists():
+        try:
+            return float(json.loads(f.read_text())["bytes_per_launch"])
+        except (ValueError, KeyError):
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--temperature", type=float, default=T_C)
+    ap.add_argument("--kernel-sweeps", type=int, default=100, help="sweeps timed kernel by kernel with HIP events")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run a device copy of BYTES bytes (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: this engine shards independent lattices per GPU (no collectives); "
+              "multi-GPU runs are not part of this build.", file=sys.stderr)
+        return 2
+
+    import computational_physics_amd as cpa
+
+    n = args.n
+    sites = n * n
+    info = cpa.device_info()
+    copy_ms = cpa.copy_calibration(args.copy_calibration, 3) if args.copy_calibration else None
+    eng = cpa.Engine(cpa.LatticeConfig(n, n, args.temperature, seed=0, spin_seed=1))
+    eng.sweep(args.warmup)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.sweep(args.steps)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    value = args.steps * sites / dt
+    ms_per_step = dt * 1e3 / args.steps
+
+    # per-kernel device time with HIP events on the engine stream
+    k_ms = eng.time_kernels(args.kernel_sweeps)
+    flags = eng.error_flags()
+    eng.close()
+    bond_label_s = k_ms[0] * 1e-3
+    achieved = BOND_LABEL_BYTES_PER_SITE * sites / bond_label_s / 1e9
+    sweep_gbps = SWEEP_BYTES_PER_SITE * sites * args.steps / dt / 1e9
+
+    result = {
+        "metric": "spin-updates/s at 4096^2 T_c (plain)",
+        "value": value,
+        "unit": "spin-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (hot start from srand48(1), lcg32 seed 0)",
+        "config": {"workload": f"plain Swendsen-Wang {n}x{n} periodic, T={args.temperature}, int8 spins, uint32 labels, "
+                               f"per-site lcg32, tile 64x16", "n0": n, "n1": n, "temperature": args.temperature},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": load_traffic(n),
+                     "kernel": "k_bond_label", "bytes_per_site": BOND_LABEL_BYTES_PER_SITE,
+                     "avg_launch_ms": k_ms[0]},
+        "sweep_roofline": {"bytes_per_site": SWEEP_BYTES_PER_SITE, "achieved_GBps": sweep_gbps,
+                           "frac": sweep_gbps / HBM_PEAK_GBPS},
+        "kernels_ms": {"k_bond_label": k_ms[0], "k_merge": k_ms[1], "k_flip": k_ms[2]},
+        "device": info["name"] or info["arch"],
+        "copy_calibration": None if copy_ms is None else {"bytes": args.copy_calibration, "ms": copy_ms,
+                                                          "GBps": 2 * args.copy_calibration / copy_ms / 1e6},
+        "error_flags": flags,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(n, args.temperature)
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

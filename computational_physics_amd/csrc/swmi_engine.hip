@@ -1,7 +1,7 @@
 // swmi_engine.hip -- host side of the C ABI (include/swmi.h).
 //
 // An engine owns a set of lattices (one for the plain path, many for a batch), their device buffers,
-// one HIP stream and the tile work list over all of them. Each sweep is three launches over the whole
+// one HIP stream and the block work list over all of them. Each sweep is three launches over the whole
 // work list, so a batch of lattices of any sizes costs the same number of launches as one lattice.
 // Errors never cross the ABI as exceptions: every entry point returns a status and records a message
 // for swmi_last_error().
@@ -114,14 +114,15 @@ namespace
         swmi_lattice_config cfg{};
         uint64_t n = 0;
         uint32_t n_groups = 0;
-        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, tiles_y = 0, nwr_y = 0, n_tiles = 0;
+        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0;
         float p_add = 0.0f;
         uint64_t threshold = 0;
         DeviceBuffer<uint8_t> spins;
         DeviceBuffer<uint32_t> rng;
         DeviceBuffer<uint64_t> masks;
         DeviceBuffer<uint32_t> rot;
-        DeviceBuffer<uint32_t> labels;
+        DeviceBuffer<uint16_t> local;
+        DeviceBuffer<uint32_t> parent;
         DeviceBuffer<uint64_t> bbits;
         DeviceBuffer<uint32_t> state;
         DeviceBuffer<int32_t> partials;
@@ -137,12 +138,13 @@ struct swmi_engine
     std::vector<std::unique_ptr<Lattice>> lattices;
     std::vector<swmi::LatDesc> descs;
     DeviceBuffer<swmi::LatDesc> d_desc;
-    DeviceBuffer<uint32_t> d_tile_lat;
-    uint32_t total_tiles = 0;
+    DeviceBuffer<uint32_t> d_block_lat;
+    uint32_t total_blocks = 0;
     uint32_t max_twr = 0;
+    bool all_tw64 = true;
     // single-lattice work lists for the inspection hooks
     DeviceBuffer<swmi::LatDesc> d_one_desc;
-    DeviceBuffer<uint32_t> d_zero_tiles;
+    DeviceBuffer<uint32_t> d_zero_blocks;
 
     ~swmi_engine()
     {
@@ -160,14 +162,15 @@ struct swmi_engine
         return *lattices[i];
     }
 
-    swmi::LatDesc Desc(const Lattice& l, uint32_t tile_base) const
+    swmi::LatDesc Desc(const Lattice& l, uint32_t block_base) const
     {
         swmi::LatDesc d{};
         d.spins = l.spins.ptr;
         d.rng = l.rng.ptr;
         d.masks = l.masks.ptr;
         d.rot = l.rot.ptr;
-        d.labels = l.labels.ptr;
+        d.local = l.local.ptr;
+        d.parent = l.parent.ptr;
         d.bbits = l.bbits.ptr;
         d.state = l.state.ptr;
         d.bond_dump = l.bond_dump.ptr;
@@ -179,11 +182,11 @@ struct swmi_engine
         d.rpw = l.rpw;
         d.twr = l.twr;
         d.tiles_x = l.tiles_x;
-        d.tiles_y = l.tiles_y;
+        d.blocks_y = l.blocks_y;
         d.nwr_y = l.nwr_y;
         d.threshold = static_cast<uint32_t>(l.threshold);
-        d.tile_base = tile_base;
-        d.n_tiles = l.n_tiles;
+        d.block_base = block_base;
+        d.n_blocks = l.n_blocks;
         d.flip_rule = l.cfg.flip_rule;
         return d;
     }
@@ -193,23 +196,25 @@ struct swmi_engine
         descs.clear();
         uint32_t base = 0;
         max_twr = 1;
-        std::vector<uint32_t> tile_lat;
+        all_tw64 = true;
+        std::vector<uint32_t> block_lat;
         for (uint32_t i = 0; i < lattices.size(); ++i)
         {
             const Lattice& l = *lattices[i];
             descs.push_back(Desc(l, base));
-            tile_lat.insert(tile_lat.end(), l.n_tiles, i);
-            base += l.n_tiles;
+            block_lat.insert(block_lat.end(), l.n_blocks, i);
+            base += l.n_blocks;
             max_twr = std::max(max_twr, l.twr);
+            all_tw64 = all_tw64 && l.tw == 64;
         }
-        total_tiles = base;
+        total_blocks = base;
         if (d_desc.count != descs.size())
             d_desc.allocate(descs.size());
-        if (d_tile_lat.count != tile_lat.size())
-            d_tile_lat.allocate(tile_lat.size());
+        if (d_block_lat.count != block_lat.size())
+            d_block_lat.allocate(block_lat.size());
         Check(hipMemcpyAsync(d_desc.ptr, descs.data(), descs.size() * sizeof(swmi::LatDesc), hipMemcpyHostToDevice, stream),
               "descriptor upload");
-        Check(hipMemcpyAsync(d_tile_lat.ptr, tile_lat.data(), tile_lat.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream),
+        Check(hipMemcpyAsync(d_block_lat.ptr, block_lat.data(), block_lat.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream),
               "work list upload");
         Check(hipStreamSynchronize(stream), "descriptor upload");
     }
@@ -220,31 +225,31 @@ struct swmi_engine
     {
         for (uint32_t i = 0; i < n; ++i)
         {
-            swmi::LaunchBondLabel(d_desc.ptr, d_tile_lat.ptr, total_tiles, max_twr, stream);
-            swmi::LaunchMerge(d_desc.ptr, d_tile_lat.ptr, total_tiles, stream);
-            swmi::LaunchFlip(d_desc.ptr, d_tile_lat.ptr, total_tiles, swmi::kFlip | swmi::kAdvance, stream);
+            swmi::LaunchBondLabel(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, all_tw64, stream);
+            swmi::LaunchMerge(d_desc.ptr, d_block_lat.ptr, total_blocks, stream);
+            swmi::LaunchFlip(d_desc.ptr, d_block_lat.ptr, total_blocks, swmi::kFlip | swmi::kAdvance, stream);
         }
         LaunchCheck("sweep launch");
     }
 
     void Sample()
     {
-        swmi::LaunchObservables(d_desc.ptr, d_tile_lat.ptr, total_tiles, stream);
+        swmi::LaunchObservables(d_desc.ptr, d_block_lat.ptr, total_blocks, stream);
         swmi::LaunchFinalizeSamples(d_desc.ptr, static_cast<uint32_t>(lattices.size()), stream);
         LaunchCheck("sample launch");
     }
 
-    // Work list holding only lattice i (tile_base 0), for the inspection hooks.
+    // Work list holding only lattice i (block_base 0), for the inspection hooks.
     void OneLattice(const Lattice& l, const uint8_t* bond_in)
     {
         swmi::LatDesc d = Desc(l, 0);
         d.bond_in = bond_in;
         if (!d_one_desc.ptr)
             d_one_desc.allocate(1);
-        if (d_zero_tiles.count < l.n_tiles)
+        if (d_zero_blocks.count < l.n_blocks)
         {
-            d_zero_tiles.allocate(l.n_tiles);
-            Check(hipMemsetAsync(d_zero_tiles.ptr, 0, l.n_tiles * sizeof(uint32_t), stream), "work list");
+            d_zero_blocks.allocate(l.n_blocks);
+            Check(hipMemsetAsync(d_zero_blocks.ptr, 0, l.n_blocks * sizeof(uint32_t), stream), "work list");
         }
         Check(hipMemcpyAsync(d_one_desc.ptr, &d, sizeof(d), hipMemcpyHostToDevice, stream), "descriptor upload");
         Check(hipStreamSynchronize(stream), "descriptor upload");
@@ -285,8 +290,8 @@ namespace
         twr = std::min<uint32_t>(twr, l.tw);      // lattice rows per tile <= 64 (right-edge bit mask)
         twr = std::min<uint32_t>(twr, l.nwr_y);
         l.twr = std::max<uint32_t>(twr, 1u);
-        l.tiles_y = (l.nwr_y + l.twr - 1) / l.twr;
-        l.n_tiles = l.tiles_x * l.tiles_y;
+        l.blocks_y = (l.nwr_y + swmi::kWavesPerBlock * l.twr - 1) / (swmi::kWavesPerBlock * l.twr);
+        l.n_blocks = l.tiles_x * l.blocks_y;
         l.p_add = BondProbability(c.temperature);
         l.threshold = BondThreshold(l.p_add);
 
@@ -294,17 +299,19 @@ namespace
         l.rng.allocate(l.n);
         l.masks.allocate(3ull * l.n_groups);
         l.rot.allocate(l.n_groups);
-        l.labels.allocate(l.n);
-        l.bbits.allocate(2ull * l.n_tiles);
+        l.local.allocate(l.n);
+        l.parent.allocate(l.n);
+        l.bbits.allocate(static_cast<size_t>(swmi::kBbitWords) * l.n_blocks);
         const size_t state_words = swmi::kLatStateWords + 4ull * c.max_samples;
         l.state.allocate(state_words);
-        l.partials.allocate(2ull * l.n_tiles);
+        l.partials.allocate(2ull * swmi::kWavesPerBlock * l.n_blocks);
         Check(hipMemsetAsync(l.state.ptr, 0, state_words * 4, stream), "state init");
         Check(hipMemcpyAsync(reinterpret_cast<uint8_t*>(l.state.ptr) + 8, &c.max_samples, 4, hipMemcpyHostToDevice, stream),
               "state init");
-        swmi::LaunchInitRng(l.rng.ptr, l.masks.ptr, l.rot.ptr, l.n_groups, c.seed, stream);
         swmi::LaunchHotStart(l.spins.ptr, l.n, c.spin_seed, stream);
-        swmi::LaunchIota(l.labels.ptr, l.n, stream);
+        swmi::LaunchInitRng(l.spins.ptr, l.rng.ptr, l.masks.ptr, l.rot.ptr, l.n_groups, c.seed, stream);
+        swmi::LaunchIota(l.parent.ptr, l.n, stream);
+        Check(hipMemsetAsync(l.local.ptr, 0, l.n * sizeof(uint16_t), stream), "label init");
         Check(hipGetLastError(), "initialisation launch");
     }
 
@@ -400,11 +407,14 @@ int swmi_upload_spins(swmi_engine* engine, uint32_t lattice, const uint8_t* spin
         if (!engine || !spins)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = engine->At(lattice);
-        std::vector<uint8_t> tmp(spins, spins + l.n);
-        for (auto& v : tmp)
-            if (v > 1)
+        for (uint64_t i = 0; i < l.n; ++i)
+            if (spins[i] > 1)
                 throw SwmiError(SWMI_ERR_INVALID, "spins must be 0 or 1");
-        Check(hipMemcpyAsync(l.spins.ptr, tmp.data(), l.n, hipMemcpyHostToDevice, engine->stream), "spin upload");
+        DeviceBuffer<uint8_t> tmp;
+        tmp.allocate(l.n);
+        Check(hipMemcpyAsync(tmp.ptr, spins, l.n, hipMemcpyHostToDevice, engine->stream), "spin upload");
+        swmi::LaunchSetSpins(l.spins.ptr, tmp.ptr, l.n, engine->stream);
+        engine->LaunchCheck("spin upload");
         Check(hipStreamSynchronize(engine->stream), "spin upload");
     });
 }
@@ -417,6 +427,8 @@ int swmi_download_spins(swmi_engine* engine, uint32_t lattice, uint8_t* spins)
         Lattice& l = engine->At(lattice);
         Check(hipMemcpyAsync(spins, l.spins.ptr, l.n, hipMemcpyDeviceToHost, engine->stream), "spin download");
         Check(hipStreamSynchronize(engine->stream), "spin download");
+        for (uint64_t i = 0; i < l.n; ++i)   // bits 1-3 carry the lcg32 selectors
+            spins[i] &= 1u;
     });
 }
 
@@ -489,13 +501,13 @@ int swmi_observables(swmi_engine* engine, uint32_t lattice, int64_t* energy, int
         if (!engine)
             throw SwmiError(SWMI_ERR_INVALID, "null engine");
         Lattice& l = engine->At(lattice);
-        swmi::LaunchObservables(engine->d_desc.ptr, engine->d_tile_lat.ptr, engine->total_tiles, engine->stream);
+        swmi::LaunchObservables(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->stream);
         engine->LaunchCheck("observables launch");
-        std::vector<int32_t> p(2ull * l.n_tiles);
+        std::vector<int32_t> p(2ull * swmi::kWavesPerBlock * l.n_blocks);
         Check(hipMemcpyAsync(p.data(), l.partials.ptr, p.size() * 4, hipMemcpyDeviceToHost, engine->stream), "observables");
         Check(hipStreamSynchronize(engine->stream), "observables");
         int64_t e = 0, m = 0;
-        for (uint32_t i = 0; i < l.n_tiles; ++i)
+        for (size_t i = 0; i < p.size() / 2; ++i)
         {
             e += p[2 * i];
             m += p[2 * i + 1];
@@ -604,9 +616,9 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
             l.bond_in.allocate(l.n);
         Check(hipMemcpyAsync(l.bond_in.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
         engine->OneLattice(l, l.bond_in.ptr);
-        swmi::LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_tiles.ptr, l.n_tiles, l.twr, engine->stream);
-        swmi::LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_tiles.ptr, l.n_tiles, engine->stream);
-        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_tiles.ptr, l.n_tiles, swmi::kWriteLabels, engine->stream);
+        swmi::LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, l.tw == 64, engine->stream);
+        swmi::LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, engine->stream);
+        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, swmi::kWriteLabels, engine->stream);
         engine->LaunchCheck("label launch");
         Check(hipStreamSynchronize(engine->stream), "label");
     });
@@ -619,9 +631,9 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = engine->At(lattice);
         engine->OneLattice(l, nullptr);
-        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_tiles.ptr, l.n_tiles, swmi::kWriteLabels, engine->stream);
+        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, swmi::kWriteLabels, engine->stream);
         engine->LaunchCheck("resolve launch");
-        Check(hipMemcpyAsync(labels, l.labels.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
+        Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
         Check(hipStreamSynchronize(engine->stream), "label download");
     });
 }
@@ -667,11 +679,11 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
         for (uint32_t i = 0; i < n_sweeps; ++i)
         {
             Check(hipEventRecord(ev[0], engine->stream), "event");
-            swmi::LaunchBondLabel(engine->d_desc.ptr, engine->d_tile_lat.ptr, engine->total_tiles, engine->max_twr, engine->stream);
+            swmi::LaunchBondLabel(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->max_twr, engine->all_tw64, engine->stream);
             Check(hipEventRecord(ev[1], engine->stream), "event");
-            swmi::LaunchMerge(engine->d_desc.ptr, engine->d_tile_lat.ptr, engine->total_tiles, engine->stream);
+            swmi::LaunchMerge(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->stream);
             Check(hipEventRecord(ev[2], engine->stream), "event");
-            swmi::LaunchFlip(engine->d_desc.ptr, engine->d_tile_lat.ptr, engine->total_tiles, swmi::kFlip | swmi::kAdvance, engine->stream);
+            swmi::LaunchFlip(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, swmi::kFlip | swmi::kAdvance, engine->stream);
             Check(hipEventRecord(ev[3], engine->stream), "event");
             Check(hipEventSynchronize(ev[3]), "event sync");
             for (int k = 0; k < 3; ++k)

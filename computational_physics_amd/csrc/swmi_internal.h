@@ -1,19 +1,22 @@
 // swmi_internal.h -- device data layout shared by the kernels and the engine.
 //
 // HBM layout of one lattice (n = n0*n1 sites, n % 64 == 0, groups of 64 sites):
-//   spins   uint8[n]        {0,1}, row-major
-//   rng     uint32[n]       per-site lcg32 state (lane i of group g = site 64g+i)
-//   masks   uint64[3*n/64]  initial (a,c) selector of each lane as three bit planes
-//   rot     uint32[n/64]    rotation offset of each group's (a,c) arrays
-//   labels  uint32[n]       union-find parents; after a sweep, site -> canonical (minimum) site index
-//   bbits   uint64[2*tiles] tile-boundary bond bits (down edge, right edge)
+//   spins    uint8[n]        bit 0: spin {0,1}; bits 1-3: the site's current lcg32 parameter selector
+//   rng      uint32[n]       per-site lcg32 state (lane i of group g = site 64g+i)
+//   masks    uint64[3*n/64]  initial (a,c) selector of each lane as three bit planes (read-only)
+//   rot      uint32[n/64]    rotation offset of each group's (a,c) arrays (updated on rotating sweeps)
+//   local    uint16[n]       block-local root of each site | 0x8000 if its cluster reaches the block edge
+//   parent   uint32[n]       global union-find over the roots of edge-reaching clusters (other entries
+//                            unused); after a resolve pass, site -> canonical (minimum) site index
+//   bbits    uint64[5*blocks] block-edge bond bits: down edge, then right edge (64 rows per word)
 //
 // Tiling. A "wave row" is 64 consecutive row-major sites, which is exactly one lcg32 group:
 //   n0 >= 64 (n0 % 64 == 0): a wave row is 64 columns of one lattice row;
 //   n0 <  64 (64 % n0 == 0): a wave row holds rpw = 64/n0 whole lattice rows.
-// A tile is `twr` wave rows stacked vertically (tw = min(n0,64) columns wide) and is labelled by one
-// wavefront: lane l of wave row r is local site r*64 + l. Local order is row-major, so the local
-// minimum maps to the global minimum.
+// A tile is `twr` wave rows (tw = min(n0,64) columns wide) and is labelled by one wavefront; a block
+// is kWavesPerBlock tiles stacked vertically and is labelled by one workgroup. Block-local index
+// q = (wave row within the block)*64 + lane is row-major, so the local minimum maps to the global
+// minimum: site(q) = block_base + (q >> 6)*stride + (q & 63), stride = 64*tiles_x.
 #pragma once
 
 #include <cstdint>
@@ -21,8 +24,11 @@
 namespace swmi
 {
     constexpr uint32_t kWave = 64;
-    constexpr uint32_t kWavesPerBlock = 4;   // tiles per workgroup
+    constexpr uint32_t kWavesPerBlock = 4;   // tiles per block (workgroup)
     constexpr uint32_t kNumXcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
+    constexpr uint16_t kEdgeFlag = 0x8000u;  // cluster reaches the block edge
+    constexpr uint32_t kRightWords = kWavesPerBlock;
+    constexpr uint32_t kBbitWords = 1 + kRightWords;
 
     struct LatDesc
     {
@@ -30,8 +36,9 @@ namespace swmi
         uint32_t* rng;
         const uint64_t* masks;
         uint32_t* rot;
-        uint32_t* labels;
-        uint64_t* bbits;
+        uint16_t* local;
+        uint32_t* parent;
+        uint64_t* bbits;         // kBbitWords words per block
         uint32_t* state;         // per-lattice device counters, see LatState
         uint8_t* bond_dump;      // per-site bond bytes of the last sweep (nullptr: not recorded)
         const uint8_t* bond_in;  // injected bond bytes (label-only mode), nullptr otherwise
@@ -41,11 +48,11 @@ namespace swmi
         uint32_t rpw;            // lattice rows per wave row: 64 / tw
         uint32_t twr;            // wave rows per tile
         uint32_t tiles_x;        // n0 / tw
-        uint32_t tiles_y;        // ceil(nwr_y / twr)
+        uint32_t blocks_y;       // ceil(nwr_y / (kWavesPerBlock * twr))
         uint32_t nwr_y;          // wave rows per column of tiles: n1 / rpw
         uint32_t threshold;      // bond iff draw u < threshold
-        uint32_t tile_base;      // first tile of this lattice in the engine work list
-        uint32_t n_tiles;
+        uint32_t block_base;     // first block of this lattice in the engine work list
+        uint32_t n_blocks;       // tiles_x * blocks_y
         uint32_t flip_rule;
     };
 
@@ -64,22 +71,25 @@ namespace swmi
     enum KernelFlags : uint32_t
     {
         kFlip = 1u,          // flip spins by root parity
-        kWriteLabels = 2u,   // write canonical labels back
+        kWriteLabels = 2u,   // write canonical labels of every site into parent[]
         kAdvance = 4u        // advance the lcg32 iteration count by 2 (end of a sweep)
     };
 
-    // Launch wrappers (swmi_kernels.hip). Work lists: tile_lat[t] = lattice of global tile t.
-    void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_tile_lat, uint32_t total_tiles, uint32_t max_twr,
-                         hipStream_t stream);
-    void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_tile_lat, uint32_t total_tiles, hipStream_t stream);
-    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_tile_lat, uint32_t total_tiles, uint32_t flags,
+    constexpr uint32_t kErrLabelRange = 1u;  // a label chase left [0, n) or met an unflagged root
+
+    // Launch wrappers (swmi_kernels.hip). Work list: block_lat[b] = lattice of global block b.
+    // all_tw64: every lattice of the work list has n0 % 64 == 0 (64-wide tiles); selects the specialised kernel
+    void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
+                         bool all_tw64, hipStream_t stream);
+    void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
+    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t flags,
                     hipStream_t stream);
-    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_tile_lat, uint32_t total_tiles, hipStream_t stream);
+    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
     void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, hipStream_t stream);
-    void LaunchInitRng(uint32_t* rng, uint64_t* masks, uint32_t* rot, uint32_t n_groups, uint32_t seed, hipStream_t stream);
+    void LaunchInitRng(uint8_t* spins, uint32_t* rng, uint64_t* masks, uint32_t* rot, uint32_t n_groups, uint32_t seed,
+                       hipStream_t stream);
+    void LaunchSetSpins(uint8_t* spins, const uint8_t* src, uint64_t n, hipStream_t stream);
     void LaunchHotStart(uint8_t* spins, uint64_t n, uint32_t spin_seed, hipStream_t stream);
     void LaunchCopy(const uint4* src, uint4* dst, uint64_t n16, hipStream_t stream);
     void LaunchIota(uint32_t* labels, uint64_t n, hipStream_t stream);
-
-    constexpr uint32_t kErrLabelRange = 1u;  // a label chase left [0, n) or ran longer than n steps
 }

@@ -1,13 +1,19 @@
 // swmi_kernels.hip -- gfx950 kernels of the Swendsen-Wang engine.
 //
-// One sweep = three launches over the engine's tile work list (all lattices of the engine at once):
-//   k_bond_label  per tile (one wavefront): two lcg32 updates per site, bond activation against the
-//                 integer threshold, row runs from 64-bit ballots, vertical unions in an LDS
-//                 union-find (atomicMin keeps parent <= child, so roots are cluster minima),
-//                 tile-local labels and the tile-boundary bond bits to HBM.
-//   k_merge       per tile: unions across tile edges by atomicMin on the uint32 labels in HBM.
-//   k_flip        per tile: chase each site's label to its root, spin ^= root & 1; advances the lcg32
-//                 iteration count.
+// One sweep = three launches over the engine's block work list (all lattices of the engine at once):
+//   k_bond_label  per block (one workgroup, 4 wavefronts, one tile each): two lcg32 updates per site,
+//                 bond activation against the integer threshold on 64-bit ballot masks (scalar unit),
+//                 row runs from the masks, unions in an LDS union-find (atomicMin keeps parent <= child,
+//                 so roots are cluster minima), block-local uint16 labels with an edge flag, and the
+//                 block-edge bond bits.
+// Spin bytes: bit 0 is the spin, bits 1-3 hold the site's current lcg32 parameter selector, i.e. the
+// index k of the Numerical Recipes pair (a_k, c_k) that lane i of its group uses after the group's
+// rotations (sel0[(i + rot) & 63]). The byte is read every sweep anyway, so the generator needs no other
+// per-group data except on the sweeps that rotate (one in 7.5).
+//   k_merge       per block: unions across block edges by atomicMin on the uint32 parent array in HBM,
+//                 over the roots of edge-reaching clusters only.
+//   k_flip        per block: a site's root is its block-local root, or (edge clusters) the chase of the
+//                 global parent array from it; spin ^= root & 1. Advances the lcg32 iteration count.
 // Reference behaviour: ising/src/swendsen_wang_amd_gpu.cpp:78-467 (four kernels per sweep) and the CPU
 // labeller ising/src/swendsen_wang_cpu.cpp:13-205; the kernels here are written for wave64 CDNA4.
 #include <hip/hip_runtime.h>
@@ -16,56 +22,75 @@
 
 namespace swmi
 {
-    // Numerical Recipes (a, c) pairs of random/src/lcg32.cpp:18-24.
-    __device__ __forceinline__ void LcgParams(const uint32_t k, uint32_t& a, uint32_t& c)
-    {
-        a = 29943829u;
-        c = 1013904223u;
-        a = (k == 3u) ? 32310901u : a;
-        c = (k == 3u) ? 626627237u : c;
-        a = (k == 2u) ? 2024337845u : a;
-        c = (k == 2u) ? 797082193u : c;
-        a = (k == 1u) ? 2891336453u : a;
-        c = (k == 1u) ? 1640531513u : c;
-        a = (k == 0u) ? 1372383749u : a;
-        c = (k == 0u) ? 1289706101u : c;
-    }
-
     __device__ __forceinline__ uint32_t Uniform(const uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-    // One LCG32_State<64>::Update (random/src/lcg32.cpp:51-84) of the group held by this wavefront:
-    // lane i owns state[i]; a[i] = Parameters[sel0[(i + rot) & 63]].
-    __device__ __forceinline__ uint32_t LcgUpdate(uint32_t st, uint32_t& rot, const uint32_t it, const uint32_t lane,
-                                                  const uint64_t m0, const uint64_t m1, const uint64_t m2)
+    // The buffers are reached through pointers stored in the lattice descriptor, which the compiler sees
+    // as generic (FLAT) addresses; FLAT accesses wait on both the memory and the LDS counters. Every
+    // buffer access goes through a global-address-space view instead.
+    template <typename T>
+    using gptr = __attribute__((address_space(1))) T*;
+
+    template <typename T>
+    __device__ __forceinline__ gptr<T> Glob(T* p)
     {
-        if (it % 15u == 0u)
-        {
-            const uint32_t shift = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(st), static_cast<int>(it & 63u))) +
-                ((it & 1u) ? 0u : 1u);
-            rot = (rot + shift) & 63u;
-        }
-        const uint32_t j = (lane + rot) & 63u;
-        const uint32_t k = static_cast<uint32_t>(((m0 >> j) & 1ull) | (((m1 >> j) & 1ull) << 1) | (((m2 >> j) & 1ull) << 2));
-        uint32_t a, c;
-        LcgParams(k, a, c);
-        return a * st + c;
+        return (gptr<T>)(p);
     }
 
-    // ---- union-find in LDS (parents satisfy P[x] <= x) ----
+    // bit `lane` of a wave-uniform mask, as a per-lane predicate (v_cndmask on the SGPR pair)
+    __device__ __forceinline__ bool LaneBit(const uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
+
+    // 64-bit shifts by 0..64 (a shift by 64 yields 0; the hardware would use the count modulo 64)
+    __device__ __forceinline__ uint64_t Shr(const uint64_t v, const uint32_t s) { return s >= 64u ? 0ull : v >> s; }
+    __device__ __forceinline__ uint64_t Shl(const uint64_t v, const uint32_t s) { return s >= 64u ? 0ull : v << s; }
+
+    __device__ __forceinline__ uint32_t ReadLane(const uint32_t v, const uint32_t lane)
+    {
+        return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+    }
+
+    // Numerical Recipes pair k of random/src/lcg32.cpp:18-24.
+    __device__ __forceinline__ void ParamsOf(const uint32_t k, uint32_t& a, uint32_t& c)
+    {
+        const bool b0 = (k & 1u) != 0u, b1 = (k & 2u) != 0u, b2 = (k & 4u) != 0u;
+        const uint32_t a01 = b0 ? 2891336453u : 1372383749u;
+        const uint32_t a23 = b0 ? 32310901u : 2024337845u;
+        const uint32_t c01 = b0 ? 1640531513u : 1289706101u;
+        const uint32_t c23 = b0 ? 626627237u : 797082193u;
+        a = b2 ? 29943829u : (b1 ? a23 : a01);
+        c = b2 ? 1013904223u : (b1 ? c23 : c01);
+    }
+
+    // Offset added to the rotation at update `it` (random/src/lcg32.cpp:56-64): state[it & 63] + (it even).
+    __device__ __forceinline__ uint32_t ShuffleShift(const uint32_t st, const uint32_t it)
+    {
+        return ReadLane(st, it & 63u) + ((it & 1u) ? 0u : 1u);
+    }
+
+    // ---- union-find in LDS (parents satisfy P[x] <= x; the workgroup shares one array) ----
     __device__ __forceinline__ uint32_t LdsLoad(const uint32_t* p)
     {
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
-    __device__ __forceinline__ uint32_t LdsFind(const uint32_t* P, uint32_t x)
+    __device__ __forceinline__ void LdsMin(uint32_t* p, const uint32_t v)
     {
-        uint32_t p = LdsLoad(&P[x]);
-        while (p != x)
+        (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+
+    // find with path halving (monotone: the shortcut is written with atomicMin)
+    __device__ __forceinline__ uint32_t LdsFind(uint32_t* P, uint32_t x)
+    {
+        for (;;)
         {
-            x = p;
-            p = LdsLoad(&P[x]);
+            const uint32_t p = LdsLoad(&P[x]);
+            if (p == x)
+                return x;
+            const uint32_t g = LdsLoad(&P[p]);
+            if (g == p)
+                return p;
+            LdsMin(&P[x], g);
+            x = g;
         }
-        return x;
     }
 
     // Same loop as the reference Merge (swendsen_wang_amd_gpu.cpp:368-401), started from roots.
@@ -83,42 +108,51 @@ namespace swmi
                 a = b;
                 b = t;
             }
-            const uint32_t old = __hip_atomic_fetch_min(&P[b], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const uint32_t old = __hip_atomic_fetch_min(&P[b], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == b)
                 return;
             b = old;
         }
     }
 
-    // ---- union-find in HBM (agent scope: the 8 XCD L2s are not coherent with each other) ----
-    __device__ __forceinline__ uint32_t GlobalLoad(const uint32_t* p)
+    // ---- union-find in HBM over the roots of edge-reaching clusters. Links are made with agent-scope
+    // atomicMin (the 8 XCD L2s are not coherent with each other for plain stores). Chases read through
+    // the caches (workgroup-scope relaxed loads: never hoisted by the compiler, allowed to hit L1/L2); a
+    // stale value is an earlier parent, i.e. still an ancestor, and every link is decided by an atomicMin
+    // that returns the true value, so staleness only costs retries. ----
+    __device__ __forceinline__ uint32_t CachedLoad(gptr<uint32_t> p)
     {
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
     // Parents satisfy L[x] <= x, so a chase ends within n steps; a value outside [0, n) can only come
     // from a broken invariant, which is reported through the lattice's error flag instead of being
-    // dereferenced.
-    __device__ __forceinline__ bool GlobalFind(const uint32_t* L, uint32_t& x, const uint32_t n)
+    // dereferenced. A start node more than one hop from its root is pointed at the root.
+    __device__ __forceinline__ bool GlobalFind(gptr<uint32_t> L, uint32_t& x, const uint32_t n)
     {
-        uint32_t p = GlobalLoad(&L[x]);
+        const uint32_t x0 = x;
+        uint32_t p = CachedLoad(&L[x]);
+        uint32_t hops = 0;
         while (p != x)
         {
             if (p >= n || p > x)
                 return false;
             x = p;
-            p = GlobalLoad(&L[x]);
+            p = CachedLoad(&L[x]);
+            ++hops;
         }
+        if (hops > 1)
+            (void)__hip_atomic_fetch_min(&L[x0], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return true;
     }
 
-    __device__ void GlobalUnion(uint32_t* L, uint32_t a, uint32_t b, const uint32_t n, uint32_t* err)
+    __device__ void GlobalUnion(gptr<uint32_t> L, uint32_t a, uint32_t b, const uint32_t n, gptr<uint32_t> err)
     {
         for (;;)
         {
             if (a >= n || b >= n || !GlobalFind(L, a, n) || !GlobalFind(L, b, n))
             {
-                atomicOr(err, kErrLabelRange);
+                (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return;
             }
             if (a == b)
@@ -137,238 +171,468 @@ namespace swmi
     }
 
     // ---- work decomposition ----
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of tiles
-    // so that tiles that share cache lines (spin rows, halos) meet in the same L2.
-    __device__ __forceinline__ bool TileOfWave(const uint32_t nblocks, const uint32_t total_tiles, uint32_t& t)
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of blocks
+    // so that blocks that share cache lines (spin rows, halos) meet in the same L2.
+    __device__ __forceinline__ uint32_t RemapBlock(const uint32_t units)
     {
-        const uint32_t per = (nblocks + kNumXcd - 1) / kNumXcd;
-        const uint32_t b = (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
-        if (b >= nblocks)
-            return false;
-        t = Uniform(b * kWavesPerBlock + (threadIdx.x >> 6));
-        return t < total_tiles;
+        const uint32_t per = (units + kNumXcd - 1) / kNumXcd;
+        return (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
     }
 
-    struct Tile
+    struct Block
     {
-        uint32_t lt;       // tile index within its lattice
-        uint32_t sx;       // tile column
-        uint32_t wy0;      // first wave row
-        uint32_t nrow;     // wave rows in this tile
+        uint32_t lb;       // block index within its lattice
+        uint32_t sx, sy;   // block column, block row
+        uint32_t wyb;      // first wave row of the block
+        uint32_t nwr;      // wave rows in the block
         uint32_t stride;   // sites between consecutive wave rows (64 * tiles_x)
-        uint32_t base0;    // first site of the tile
+        uint32_t base;     // first site of the block
     };
 
-    __device__ __forceinline__ Tile DecodeTile(const LatDesc& d, const uint32_t t)
+    __device__ __forceinline__ Block DecodeBlock(const LatDesc& d, const uint32_t b)
     {
-        Tile tl;
-        tl.lt = t - d.tile_base;
-        tl.sx = tl.lt % d.tiles_x;
-        tl.wy0 = (tl.lt / d.tiles_x) * d.twr;
-        tl.nrow = min(d.twr, d.nwr_y - tl.wy0);
-        tl.stride = 64u * d.tiles_x;
-        tl.base0 = tl.wy0 * tl.stride + tl.sx * 64u;
-        return tl;
+        Block bl;
+        bl.lb = b - d.block_base;
+        bl.sx = bl.lb % d.tiles_x;
+        bl.sy = bl.lb / d.tiles_x;
+        bl.wyb = bl.sy * kWavesPerBlock * d.twr;
+        bl.nwr = min(kWavesPerBlock * d.twr, d.nwr_y - bl.wyb);
+        bl.stride = 64u * d.tiles_x;
+        bl.base = bl.wyb * bl.stride + bl.sx * 64u;
+        return bl;
+    }
+
+    __device__ __forceinline__ uint32_t BlockBase(const LatDesc& d, const uint32_t sx, const uint32_t sy)
+    {
+        return sy * kWavesPerBlock * d.twr * 64u * d.tiles_x + sx * 64u;
+    }
+
+    __device__ __forceinline__ uint32_t SiteOf(const uint32_t base, const uint32_t stride, const uint32_t q)
+    {
+        return base + (q >> 6) * stride + (q & 63u);
     }
 
     // =============================================================================================
     // k_bond_label
     // =============================================================================================
-    template <uint32_t TWR_MAX>
-    __global__ __launch_bounds__(256) void k_bond_label(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ tile_lat,
-                                                        const uint32_t total_tiles, const uint32_t nblocks)
+    // Down bonds are deduplicated before any union: lanes l-1 and l lie in the same "overlap interval"
+    // when they share their row run above and their row run below (mask C); only the first down bond
+    // of every interval can join two different clusters. With M = C << 1 (continuation) and
+    // S1 = (D << 1) & M, the positions that follow an earlier bond of the same interval are
+    // S1 | (M & ((M + S1) ^ M)) (carries of S1 run through the ones of M).
+    __device__ __forceinline__ uint64_t FirstBondPerInterval(const uint64_t D, const uint64_t C, const uint64_t rowstart)
     {
-        __shared__ uint32_t lds_parent[kWavesPerBlock][TWR_MAX * 64];
-        uint32_t t;
-        if (!TileOfWave(nblocks, total_tiles, t))
-            return;
-        const uint32_t lane = threadIdx.x & 63u;
-        const LatDesc& d = descs[tile_lat[t]];
-        const Tile tl = DecodeTile(d, t);
-        uint32_t* P = lds_parent[threadIdx.x >> 6];
+        const uint64_t M = (C << 1) & ~rowstart;
+        const uint64_t S1 = (D << 1) & M;
+        const uint64_t later = S1 | (M & ((M + S1) ^ M));
+        return D & ~later;
+    }
 
-        const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw;
+    // Branch-free union of the pairs (a, b) held by the lanes, all lanes in lockstep. Each step moves
+    // both ends one parent up; when both are roots the larger is linked under the smaller with
+    // atomicMin, and a lost race continues with the value found (the loop of the reference Merge,
+    // swendsen_wang_amd_gpu.cpp:368-401). Lanes with nothing to do hold a == b; for them, and for ends
+    // that are not roots yet, the atomic is atomicMin(P[hi], hi), which never changes P (P[x] <= x).
+    __device__ __forceinline__ void LdsUnionLockstep(uint32_t* P, uint32_t a, uint32_t b)
+    {
+        while (__ballot(a != b))
+        {
+            const uint32_t pa = LdsLoad(&P[a]);
+            const uint32_t pb = LdsLoad(&P[b]);
+            const bool roots = (pa == a) && (pb == b);
+            const uint32_t lo = min(a, b), hi = max(a, b);
+            const uint32_t old = __hip_atomic_fetch_min(&P[hi], (roots && a != b) ? lo : hi, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            const bool linked = roots && (a == b || old == hi);
+            a = roots ? (linked ? hi : lo) : pa;
+            b = roots ? (linked ? hi : old) : pb;
+        }
+    }
+
+    // Per wavefront (tile) the rows are processed top to bottom with the loads of the next kCh rows in
+    // flight together. Bonds are computed on wave-uniform 64-bit masks: S (spin up), U1/U2 (draw below
+    // threshold), B (bond to the right neighbour inside the tile), D (bond to the site below). Rows
+    // only record their unions (one per overlap interval) in a per-wave LDS queue, which is worked off
+    // 64 unions at a time whenever it fills (unions only involve rows already initialised).
+    template <uint32_t TWR_MAX, bool TW64>
+    __global__ __launch_bounds__(256) void k_bond_label(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                        const uint32_t total_blocks)
+    {
+        // All loads of a chunk of rows are issued before its first store: on this hardware the memory
+        // counter that a load waits on also counts the stores issued before it.
+        constexpr uint32_t kCh = TWR_MAX < 16 ? TWR_MAX : 16;
+        constexpr uint32_t kRows = TWR_MAX < 16 ? TWR_MAX : 16;   // rows resolved together at the end
+        constexpr uint32_t kSites = kWavesPerBlock * TWR_MAX * 64;
+        __shared__ uint32_t P[kSites];
+        __shared__ uint32_t Q[kWavesPerBlock * 128];   // per-wave union queues: (a << 16) | b
+        __shared__ uint32_t F[kSites / 32];     // roots of clusters that reach the block edge
+        __shared__ uint64_t s_first_B[kWavesPerBlock];
+
+        const uint32_t b = RemapBlock(total_blocks);
+        if (b >= total_blocks)
+            return;
+        const LatDesc& d = descs[block_lat[b]];
+        const Block bl = DecodeBlock(d, b);
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t w = Uniform(threadIdx.x >> 6);
+
+        const uint32_t n0 = d.n0, n1 = d.n1, twr = d.twr;
+        const uint32_t tw = TW64 ? 64u : d.tw, rpw = TW64 ? 1u : d.rpw;
         const uint32_t lx = lane & (tw - 1u);
-        const uint32_t lrow = lane / tw;
-        const uint32_t x = tl.sx * tw + lx;
+        const uint32_t lrow = TW64 ? 0u : lane / tw;
+        const uint32_t x = bl.sx * tw + lx;
+        const uint32_t x_right = (x + 1u == n0) ? 0u : x + 1u;
+        const bool rbound = lx == tw - 1u;
         const bool injected = d.bond_in != nullptr;
         const uint32_t thr = d.threshold;
-        const uint32_t it0 = injected ? 0u : d.state[0];
-        uint8_t* const dump = d.bond_dump;
+        const uint32_t it1 = (injected ? 0u : Glob(d.state)[0]) + 1u;
+        const uint32_t it2 = it1 + 1u;
+        const bool rot1 = !injected && (it1 % 15u == 0u);
+        const bool rot2 = !injected && (it2 % 15u == 0u);
+        const gptr<uint8_t> dump = Glob(d.bond_dump);
+        const gptr<uint8_t> spins = Glob(d.spins);
+        const gptr<uint32_t> rng = Glob(d.rng);
+        const gptr<uint32_t> rotp = Glob(d.rot);
+        const gptr<const uint8_t> bond_in = Glob(d.bond_in);
+        const gptr<uint16_t> local = Glob(d.local);
+        const gptr<uint32_t> parent = Glob(d.parent);
 
-        // lane masks: first column of a lattice row, lanes with an in-tile right neighbour, lanes whose
-        // down neighbour sits in the same wave row
-        uint64_t rowstart = 0;
-        for (uint32_t j = 0; j < 64u; j += tw)
-            rowstart |= 1ull << j;
+        const uint32_t r_begin = w * twr;
+        const uint32_t nrow = bl.nwr > r_begin ? min(twr, bl.nwr - r_begin) : 0u;
+        const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
+        const bool last_wave = w + 1u == nwaves;
+        uint32_t* const Qw = Q + w * 128u;
+        // a full queue (>= 64 entries) is worked off right away; the tail moves to the front
+        const auto drain64 = [&](uint32_t& qlen) {
+            const uint32_t e = Qw[lane];
+            LdsUnionLockstep(P, e >> 16, e & 0xFFFFu);
+            const uint32_t rem = qlen - 64u;
+            const uint32_t t = Qw[64u + lane];
+            if (lane < rem)
+                Qw[lane] = t;
+            qlen = rem;
+        };
+
+        // lane masks: first column of a lattice row, lanes with a right neighbour inside the wave row,
+        // lanes whose down neighbour sits in the same wave row
+        uint64_t rowstart = 1;
+        if (!TW64)
+            for (uint32_t j = tw; j < 64u; j += tw)
+                rowstart |= 1ull << j;
         const uint64_t in_right = ~(rowstart << (tw - 1u));
-        const uint64_t low = (tw == 64u) ? 0ull : ((1ull << (64u - tw)) - 1ull);
-        const bool same_row_down = lane < 64u - tw;
+        const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
+        const uint64_t lanes_below = (1ull << lane) - 1ull;
 
-        uint32_t prev_s = 0, prev_u2 = 0, prev_site = 0;
-        bool prev_bx = false;
-        uint64_t prev_B = 0, right_bits = 0;
+        for (uint32_t i = threadIdx.x; i < kSites / 32; i += 256u)
+            F[i] = 0;
 
-        for (uint32_t r = 0; r < tl.nrow; ++r)
+        uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
+        uint32_t qlen = 0;
+        if (nrow > 0)
         {
-            const uint32_t row_base = tl.base0 + r * tl.stride;
-            const uint32_t site = row_base + lane;
-            const uint32_t y = (tl.wy0 + r) * rpw + lrow;
-            const uint32_t s = d.spins[site];
-            uint32_t u1 = 0, u2 = 0;
-            if (!injected)
+            // spins below the tile (down halo), needed last but fetched first
+            const uint32_t y_last = (bl.wyb + r_begin + nrow - 1u) * rpw + lrow;
+            const uint32_t halo_down = (lane < 64u - tw) ? 0u : spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
+
+            uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
+            uint32_t prev_site = 0;
+            for (uint32_t r0 = 0; r0 < nrow; r0 += kCh)
             {
-                const uint32_t grp = row_base >> 6;
-                uint32_t rot = d.rot[grp];
-                const uint32_t rot0 = rot;
-                const uint64_t m0 = d.masks[3u * grp], m1 = d.masks[3u * grp + 1u], m2 = d.masks[3u * grp + 2u];
-                uint32_t st = d.rng[site];
-                st = LcgUpdate(st, rot, it0 + 1u, lane, m0, m1, m2);
-                u1 = st;
-                st = LcgUpdate(st, rot, it0 + 2u, lane, m0, m1, m2);
-                u2 = st;
-                d.rng[site] = st;
-                if (rot != rot0 && lane == 0)
-                    d.rot[grp] = rot;
+                uint32_t sp[kCh], aux[kCh], rsh[kCh];
+#pragma unroll
+                for (uint32_t i = 0; i < kCh; ++i)
+                {
+                    const uint32_t wr = r_begin + r0 + i;
+                    sp[i] = aux[i] = rsh[i] = 0;
+                    if (r0 + i < nrow)
+                    {
+                        const uint32_t site = bl.base + wr * bl.stride + lane;
+                        sp[i] = spins[site];
+                        aux[i] = injected ? bond_in[site] : rng[site];
+                        if (!TW64 && rbound)
+                            rsh[i] = spins[((bl.wyb + wr) * rpw + lrow) * n0 + x_right];
+                    }
+                }
+                // rotation offsets of the chunk's groups (lane i: row r0 + i), needed on rotating sweeps only
+                uint32_t g_rot = 0;
+                if ((rot1 || rot2) && lane < kCh && r0 + lane < nrow)
+                    g_rot = rotp[(bl.base + (r_begin + r0 + lane) * bl.stride) >> 6];
+                // right halo of a 64-wide tile: lane i fetches the column-64 spin of row r0 + i
+                uint64_t HR = 0;
+                if (TW64)
+                {
+                    const uint32_t col = (bl.sx * 64u + 64u == n0) ? 0u : bl.sx * 64u + 64u;
+                    uint32_t h = 0;
+                    if (lane < kCh && r0 + lane < nrow)
+                        h = spins[(bl.wyb + r_begin + r0 + lane) * n0 + col];
+                    HR = __ballot((h & 1u) != 0u);
+                }
+
+#pragma unroll
+                for (uint32_t i = 0; i < kCh; ++i)
+                {
+                    const uint32_t r = r0 + i;
+                    if (r < nrow)
+                    {
+                        const uint32_t wr = r_begin + r;
+                        const uint32_t site = bl.base + wr * bl.stride + lane;
+                        const uint64_t S = __ballot((sp[i] & 1u) != 0u);
+                        uint64_t Ball, U2;
+                        if (!injected)
+                        {
+                            // two LCG32_State<64>::Update of the group of this wave row
+                            // (random/src/lcg32.cpp:51-84); a rotation by `shift` moves lane
+                            // (i + shift) & 63's selector to lane i
+                            uint32_t k = (sp[i] >> 1) & 7u, st = aux[i], a, c, shift = 0;
+                            if (rot1)
+                            {
+                                shift = ShuffleShift(st, it1) & 63u;
+                                k = static_cast<uint32_t>(__shfl(static_cast<int>(k), static_cast<int>((lane + shift) & 63u)));
+                            }
+                            ParamsOf(k, a, c);
+                            st = a * st + c;
+                            const uint32_t u1 = st;
+                            if (rot2)
+                            {
+                                const uint32_t s2 = ShuffleShift(st, it2) & 63u;
+                                shift += s2;
+                                k = static_cast<uint32_t>(__shfl(static_cast<int>(k), static_cast<int>((lane + s2) & 63u)));
+                                ParamsOf(k, a, c);
+                            }
+                            st = a * st + c;
+                            rng[site] = st;
+                            if (rot1 || rot2)
+                            {
+                                spins[site] = static_cast<uint8_t>((sp[i] & 1u) | (k << 1));
+                                if (lane == 0)
+                                    rotp[site >> 6] = (ReadLane(g_rot, i) + shift) & 63u;
+                            }
+                            const uint64_t U1 = __ballot(u1 < thr);
+                            U2 = __ballot(st < thr);
+                            // right neighbour's spin: next lane, or the halo for the last column
+                            const uint64_t RS = TW64 ? ((S >> 1) | (((HR >> i) & 1ull) << 63))
+                                                     : (((S >> 1) & in_right) | __ballot(rbound && (rsh[i] & 1u) != 0u));
+                            Ball = ~(S ^ RS) & U1;
+                        }
+                        else
+                        {
+                            Ball = __ballot((aux[i] & 1u) != 0u);
+                            U2 = __ballot((aux[i] & 2u) != 0u);
+                        }
+                        const uint64_t B = Ball & in_right;
+                        const uint64_t edge = Ball & ~in_right;
+                        if (TW64)
+                            right_bits |= (edge >> 63) << r;
+                        else
+                            for (uint32_t j = 0; j < rpw; ++j)
+                                right_bits |= ((edge >> (j * tw + tw - 1u)) & 1ull) << (r * rpw + j);
+                        if (r == 0)
+                            s_first_B[w] = B;
+
+                        // every site starts under the first site of its row run
+                        const uint64_t below = ~B & lanes_below;
+                        const uint32_t run_start = below ? 64u - static_cast<uint32_t>(__clzll(below)) : 0u;
+                        P[wr * 64u + lane] = wr * 64u + run_start;
+
+                        if (r > 0)
+                        {
+                            // bonds (x, y) -> (x, y+1) of the previous wave row: queue one union per
+                            // overlap interval
+                            const uint64_t SD = TW64 ? S : ((Shr(prev_S, tw) & low) | Shl(S, 64u - tw));
+                            const uint64_t D = injected ? prev_U2 : (~(prev_S ^ SD) & prev_U2);
+                            const uint64_t tgtB = TW64 ? B : (Shr(prev_B, tw) | Shl(B, 64u - tw));
+                            const uint64_t K = FirstBondPerInterval(D, prev_B & tgtB, rowstart);
+                            if (LaneBit(K))
+                            {
+                                const uint32_t top = (wr - 1u) * 64u + lane;
+                                const uint32_t bot = TW64 ? wr * 64u + run_start
+                                                          : ((lane < 64u - tw) ? top + tw : wr * 64u + lane - (64u - tw));
+                                const uint32_t pos = qlen + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
+                                                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
+                                Qw[pos] = (top << 16) | bot;
+                            }
+                            qlen += static_cast<uint32_t>(__popcll(K));
+                            if (qlen >= 64u)
+                                drain64(qlen);
+                            if (dump)
+                                dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
+                        }
+                        prev_S = S;
+                        prev_U2 = U2;
+                        prev_B = B;
+                        prev_Ball = Ball;
+                        prev_site = site;
+                    }
+                }
             }
 
-            // bond to (x+1, y)
-            uint32_t rs = static_cast<uint32_t>(__shfl(static_cast<int>(s), static_cast<int>(lane + 1u)));
-            const bool rbound = lx == tw - 1u;
-            if (rbound)
-                rs = d.spins[y * n0 + ((x + 1u == n0) ? 0u : x + 1u)];
-            const bool bx = injected ? ((d.bond_in[site] & 1u) != 0) : ((s == rs) && (u1 < thr));
-            const uint64_t b_all = __ballot(bx);
-            const uint64_t B = b_all & in_right;
-            const uint64_t b_edge = b_all & ~in_right;
-            if (tw == 64u)
-                right_bits |= (b_edge >> 63) << r;
-            else
-                for (uint32_t j = 0; j < rpw; ++j)
-                    right_bits |= ((b_edge >> (j * tw + tw - 1u)) & 1ull) << (r * rpw + j);
-
-            // every site starts under the first site of its row run
-            const uint64_t below = ~B & ((1ull << lane) - 1ull);
-            const uint32_t run_start = below ? 64u - static_cast<uint32_t>(__clzll(below)) : 0u;
-            P[r * 64u + lane] = r * 64u + run_start;
-
-            if (r > 0)
+            // bonds out of the last wave row: into the same wave row (tw < 64), into the next tile of the
+            // block, or across the block edge
             {
-                // bonds (x, y) -> (x, y+1) of wave row r-1
-                const uint32_t sd_same = static_cast<uint32_t>(__shfl(static_cast<int>(prev_s), static_cast<int>(lane + tw)));
-                const uint32_t sd_next = static_cast<uint32_t>(__shfl(static_cast<int>(s), static_cast<int>(lane - (64u - tw))));
-                const uint32_t sd = same_row_down ? sd_same : sd_next;
-                const bool by = injected ? ((d.bond_in[prev_site] & 2u) != 0) : ((prev_s == sd) && (prev_u2 < thr));
-                const uint64_t D = __ballot(by);
-                const uint64_t tgtB = (tw == 64u) ? B : ((prev_B >> tw) | (B << (64u - tw)));
-                const uint64_t redundant = ((D & prev_B & tgtB) << 1) & ~rowstart;
-                if (((D & ~redundant) >> lane) & 1ull)
+                const uint32_t wr = r_begin + nrow - 1u;
+                // halo spins arrive in the lanes that need them (lanes >= 64 - tw)
+                const uint64_t H = __ballot((halo_down & 1u) != 0u);
+                const uint64_t SD = TW64 ? H : ((Shr(prev_S, tw) & low) | (H & ~low));
+                const uint64_t D = injected ? prev_U2 : (~(prev_S ^ SD) & prev_U2);
+                if (!TW64)
                 {
-                    const uint32_t tq = same_row_down ? (r - 1u) * 64u + lane + tw : r * 64u + lane - (64u - tw);
-                    LdsUnion(P, (r - 1u) * 64u + lane, tq);
+                    const uint64_t Din = D & low;
+                    const uint64_t K = FirstBondPerInterval(Din, prev_B & Shr(prev_B, tw), rowstart);
+                    if (LaneBit(K))
+                    {
+                        const uint32_t pos = qlen + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
+                                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
+                        Qw[pos] = ((wr * 64u + lane) << 16) | (wr * 64u + lane + tw);
+                    }
+                    qlen += static_cast<uint32_t>(__popcll(K));
+                    if (qlen >= 64u)
+                        drain64(qlen);
                 }
                 if (dump)
-                    dump[prev_site] = static_cast<uint8_t>((prev_bx ? 1u : 0u) | (by ? 2u : 0u));
+                    dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
+                last_B = prev_B;
+                if (last_wave)
+                    down_bits = D & ~low;
+                else
+                    last_D = D & ~low;
             }
-            prev_s = s;
-            prev_u2 = u2;
-            prev_B = B;
-            prev_site = site;
-            prev_bx = bx;
-        }
 
-        // bonds out of the last wave row: into the same wave row (tw < 64) or across the tile edge
-        uint64_t down_bits;
-        {
-            const uint32_t r = tl.nrow - 1u;
-            const uint32_t y = (tl.wy0 + r) * rpw + lrow;
-            const uint32_t sd_same = static_cast<uint32_t>(__shfl(static_cast<int>(prev_s), static_cast<int>(lane + tw)));
-            uint32_t sd = sd_same;
-            if (!same_row_down)
-                sd = d.spins[((y + 1u == n1) ? 0u : y + 1u) * n0 + x];
-            const bool by = injected ? ((d.bond_in[prev_site] & 2u) != 0) : ((prev_s == sd) && (prev_u2 < thr));
-            const uint64_t D = __ballot(by);
-            const uint64_t Din = D & low;
-            down_bits = D & ~low;
-            const uint64_t tgtB = (tw == 64u) ? 0ull : (prev_B >> tw);
-            const uint64_t redundant = ((Din & prev_B & tgtB) << 1) & ~rowstart;
-            if (((Din & ~redundant) >> lane) & 1ull)
-                LdsUnion(P, r * 64u + lane, r * 64u + lane + tw);
-            if (dump)
-                dump[prev_site] = static_cast<uint8_t>((prev_bx ? 1u : 0u) | (by ? 2u : 0u));
+            // the rest of the queued unions of this tile (fewer than 64)
+            if (qlen > 0)
+            {
+                const uint32_t e = (lane < qlen) ? Qw[lane] : 0u;
+                LdsUnionLockstep(P, e >> 16, e & 0xFFFFu);
+            }
         }
+        __syncthreads();
 
-        // tile-local roots -> global site indices (monotone map)
-        for (uint32_t r = 0; r < tl.nrow; ++r)
+        // unions across the tile edges inside the block
+        if (nrow > 0 && !last_wave)
         {
-            const uint32_t q = LdsFind(P, r * 64u + lane);
-            d.labels[tl.base0 + r * tl.stride + lane] = tl.base0 + (q >> 6) * tl.stride + (q & 63u);
+            const uint32_t wr = r_begin + nrow - 1u;
+            const uint64_t Bbot = s_first_B[w + 1u];
+            const uint64_t tgtB = TW64 ? Bbot : Shl(Bbot, 64u - tw);
+            const bool go = LaneBit(FirstBondPerInterval(last_D, last_B & tgtB, rowstart));
+            LdsUnionLockstep(P, go ? wr * 64u + lane : 0u, go ? (wr + 1u) * 64u + lane - (64u - tw) : 0u);
         }
+        __syncthreads();
+
+        // every site points straight at its block-local root (kRows chases in flight per lane); sites on
+        // the block edge mark their root
+        for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
+        {
+            uint32_t xs[kRows];
+#pragma unroll
+            for (uint32_t i = 0; i < kRows; ++i)
+                xs[i] = (r0 + i < nrow) ? (r_begin + r0 + i) * 64u + lane : (r_begin + r0) * 64u + lane;
+            bool moving = true;
+            while (__ballot(moving))
+            {
+                moving = false;
+#pragma unroll
+                for (uint32_t i = 0; i < kRows; ++i)
+                {
+                    const uint32_t p = LdsLoad(&P[xs[i]]);
+                    moving = moving || (p != xs[i]);
+                    xs[i] = p;
+                }
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kRows; ++i)
+            {
+                if (r0 + i < nrow)
+                {
+                    const uint32_t wr = r_begin + r0 + i;
+                    P[wr * 64u + lane] = xs[i];
+                    const bool on_edge = lx == 0 || rbound || (wr == 0 && lrow == 0) || (wr + 1u == bl.nwr && lrow == rpw - 1u);
+                    (void)__hip_atomic_fetch_or(&F[xs[i] >> 5], on_edge ? (1u << (xs[i] & 31u)) : 0u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+        __syncthreads();
+
+        for (uint32_t r = 0; r < nrow; ++r)
+        {
+            const uint32_t q = (r_begin + r) * 64u + lane;
+            const uint32_t site = bl.base + (r_begin + r) * bl.stride + lane;
+            const uint32_t root = LdsLoad(&P[q]);
+            const bool flagged = ((LdsLoad(&F[root >> 5]) >> (root & 31u)) & 1u) != 0u;
+            local[site] = static_cast<uint16_t>(root | (flagged ? kEdgeFlag : 0u));
+            if (flagged && root == q)
+                parent[site] = site;
+        }
+        const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         if (lane == 0)
         {
-            d.bbits[2u * tl.lt] = down_bits;
-            d.bbits[2u * tl.lt + 1u] = right_bits;
+            if (last_wave)
+                bb[0] = down_bits;
+            bb[1u + w] = right_bits;
         }
     }
 
     // =============================================================================================
-    // k_merge: unions across tile edges (reference MergeLabels_Kernel, swendsen_wang_amd_gpu.cpp:280-329)
+    // k_merge: unions across block edges (reference MergeLabels_Kernel, swendsen_wang_amd_gpu.cpp:280-329)
     // =============================================================================================
-    __global__ __launch_bounds__(256) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ tile_lat,
-                                                   const uint32_t total_tiles, const uint32_t nblocks)
+    // One workgroup per block: wavefront w takes the right-edge word of tile w (one lane per lattice row),
+    // wavefront 0 also the down edge (one lane per column), so the unions of a block run in parallel.
+    __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t a_site,
+                                              const uint32_t b_site, const uint32_t base_b, const bool same_line,
+                                              gptr<uint32_t> err)
     {
-        uint32_t t;
-        if (!TileOfWave(nblocks, total_tiles, t))
-            return;
         const uint32_t lane = threadIdx.x & 63u;
-        const LatDesc& d = descs[tile_lat[t]];
-        const Tile tl = DecodeTile(d, t);
-        const uint64_t down_bits = d.bbits[2u * tl.lt];
-        const uint64_t right_bits = d.bbits[2u * tl.lt + 1u];
-        if ((down_bits | right_bits) == 0ull)
-            return;
-        uint32_t* const L = d.labels;
-        const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw;
-        const uint32_t lx = lane & (tw - 1u);
-        const uint32_t n = n0 * n1;
-        uint32_t* const err = d.state + 3;
-
-        // down edge: lanes of the last lattice row of the last wave row
+        const bool bit = LaneBit(bits);
+        const gptr<uint16_t> local = Glob(d.local);
+        uint32_t a = 0, bq = 0;
+        if (bit)
         {
-            const uint32_t r = tl.nrow - 1u;
-            const uint32_t y = (tl.wy0 + r) * rpw + lane / tw;
-            const uint32_t x = tl.sx * tw + lx;
-            const bool bit = ((down_bits >> lane) & 1ull) != 0;
-            uint32_t a = 0, b = 0;
-            if (bit)
-            {
-                a = GlobalLoad(&L[tl.base0 + r * tl.stride + lane]);
-                b = GlobalLoad(&L[((y + 1u == n1) ? 0u : y + 1u) * n0 + x]);
-            }
-            const uint32_t ap = static_cast<uint32_t>(__shfl(static_cast<int>(a), static_cast<int>(lane - 1u)));
-            const uint32_t bp = static_cast<uint32_t>(__shfl(static_cast<int>(b), static_cast<int>(lane - 1u)));
-            const bool dup = lx != 0 && ((down_bits >> (lane - 1u)) & 1ull) && ap == a && bp == b;
-            if (bit && !dup && a != b)
-                GlobalUnion(L, a, b, n, err);
+            const uint32_t va = local[a_site], vb = local[b_site];
+            if (!(va & kEdgeFlag) || !(vb & kEdgeFlag))
+                (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a = SiteOf(bl.base, bl.stride, va & 0x7FFFu);
+            bq = SiteOf(base_b, bl.stride, vb & 0x7FFFu);
         }
-        // right edge: lane k = lattice row k of the tile
+        const uint32_t ap = static_cast<uint32_t>(__shfl(static_cast<int>(a), static_cast<int>(lane - 1u)));
+        const uint32_t bp = static_cast<uint32_t>(__shfl(static_cast<int>(bq), static_cast<int>(lane - 1u)));
+        const bool dup = same_line && LaneBit(bits << 1) && ap == a && bp == bq;
+        if (bit && !dup && a != bq)
+            GlobalUnion(Glob(d.parent), a, bq, d.n0 * d.n1, err);
+    }
+
+    __global__ __launch_bounds__(256) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                   const uint32_t total_blocks)
+    {
+        const uint32_t b = RemapBlock(total_blocks);
+        if (b >= total_blocks)
+            return;
+        const LatDesc& d = descs[block_lat[b]];
+        const Block bl = DecodeBlock(d, b);
+        const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t w = Uniform(threadIdx.x >> 6);
+        const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw, twr = d.twr;
+        const uint32_t lx = lane & (tw - 1u);
+        const gptr<uint32_t> err = Glob(d.state) + 3;
+
+        // right edge of tile w: lane k = lattice row k of the tile
+        const uint64_t right_bits = bb[1u + w];
+        if (right_bits)
         {
-            const uint32_t rows = tl.nrow * rpw;
-            const uint32_t k = lane;
-            const bool bit = k < rows && ((right_bits >> k) & 1ull) != 0;
-            const uint32_t y = tl.wy0 * rpw + k;
-            const uint32_t xr = tl.sx * tw + tw - 1u;
-            uint32_t a = 0, b = 0;
-            if (bit)
-            {
-                a = GlobalLoad(&L[y * n0 + xr]);
-                b = GlobalLoad(&L[y * n0 + ((xr + 1u == n0) ? 0u : xr + 1u)]);
-            }
-            const uint32_t ap = static_cast<uint32_t>(__shfl(static_cast<int>(a), static_cast<int>(lane - 1u)));
-            const uint32_t bp = static_cast<uint32_t>(__shfl(static_cast<int>(b), static_cast<int>(lane - 1u)));
-            const bool dup = k > 0 && ((right_bits >> (k - 1u)) & 1ull) && ap == a && bp == b;
-            if (bit && !dup && a != b)
-                GlobalUnion(L, a, b, n, err);
+            const uint32_t sx_r = (bl.sx + 1u == d.tiles_x) ? 0u : bl.sx + 1u;
+            const uint32_t xr = bl.sx * tw + tw - 1u;
+            const uint32_t xr1 = (xr + 1u == n0) ? 0u : xr + 1u;
+            const uint32_t y = (bl.wyb + w * twr) * rpw + lane;
+            MergeEdge(d, bl, right_bits, y * n0 + xr, y * n0 + xr1, BlockBase(d, sx_r, bl.sy), lane > 0, err);
+        }
+        // down edge: lanes of the last lattice row of the block
+        const uint64_t down_bits = (w == 0) ? bb[0] : 0ull;
+        if (down_bits)
+        {
+            const uint32_t y = (bl.wyb + bl.nwr - 1u) * rpw + lane / tw;
+            const uint32_t x = bl.sx * tw + lx;
+            const bool wrap = y + 1u == n1;
+            MergeEdge(d, bl, down_bits, y * n0 + x, (wrap ? 0u : y + 1u) * n0 + x, BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u),
+                      lx != 0, err);
         }
     }
 
@@ -376,74 +640,162 @@ namespace swmi
     // k_flip: resolve + flip (reference ResolveLabels_Kernel / FlipClusters_Kernel,
     // swendsen_wang_amd_gpu.cpp:403-467)
     // =============================================================================================
-    __global__ __launch_bounds__(256) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ tile_lat,
-                                                  const uint32_t total_tiles, const uint32_t nblocks, const uint32_t flags)
+    // Per chunk of kCh rows: all block-local labels and spins are fetched, then the parent of every edge
+    // root, then the grandparent; a root whose chain is longer (rare: the merge points every start node
+    // at its root) is finished in a slow pass.
+    __global__ __launch_bounds__(256) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                  const uint32_t total_blocks, const uint32_t flags)
     {
-        uint32_t t;
-        if (!TileOfWave(nblocks, total_tiles, t))
+        // loads of a chunk (labels, spins, parents, grandparents) all precede its stores: the memory
+        // counter a load waits on also counts earlier stores
+        constexpr uint32_t kCh = 16;
+        const uint32_t b = RemapBlock(total_blocks);
+        if (b >= total_blocks)
             return;
+        const LatDesc& d = descs[block_lat[b]];
+        const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
-        const LatDesc& d = descs[tile_lat[t]];
-        const Tile tl = DecodeTile(d, t);
-        uint32_t* const L = d.labels;
-        for (uint32_t r = 0; r < tl.nrow; ++r)
+        const uint32_t w = Uniform(threadIdx.x >> 6);
+        const uint32_t r_begin = w * d.twr;
+        const uint32_t nrow = bl.nwr > r_begin ? min(d.twr, bl.nwr - r_begin) : 0u;
+        const gptr<uint32_t> L = Glob(d.parent);
+        const gptr<uint16_t> local = Glob(d.local);
+        const gptr<uint8_t> spins = Glob(d.spins);
+        const uint32_t n = d.n0 * d.n1;
+        const bool do_flip = (flags & kFlip) != 0u;
+        const bool write_labels = (flags & kWriteLabels) != 0u;
+        const uint32_t row0 = bl.base + r_begin * bl.stride + lane;
+        bool ok = true;
+
+        for (uint32_t r0 = 0; r0 < nrow; r0 += kCh)
         {
-            const uint32_t site = tl.base0 + r * tl.stride + lane;
-            uint32_t x = L[site];
-            bool ok = x <= site;
-            uint32_t p = ok ? L[x] : x;
-            while (p != x)
+            uint32_t loc[kCh], sp[kCh], root[kCh], par[kCh], gpar[kCh];
+#pragma unroll
+            for (uint32_t i = 0; i < kCh; ++i)
             {
-                if (p > x)
+                loc[i] = sp[i] = 0;
+                if (r0 + i < nrow)
                 {
-                    ok = false;
-                    break;
+                    const uint32_t site = row0 + (r0 + i) * bl.stride;
+                    loc[i] = local[site];
+                    if (do_flip)
+                        sp[i] = spins[site];
                 }
-                x = p;
-                p = L[x];
             }
-            if (!ok)
+            // loads are unconditional (lanes without a lookup read entry 0) so that the compiler keeps
+            // them as plain global loads
+#pragma unroll
+            for (uint32_t i = 0; i < kCh; ++i)
             {
-                atomicOr(d.state + 3, kErrLabelRange);
-                x = site;
+                root[i] = SiteOf(bl.base, bl.stride, loc[i] & 0x7FFFu);
+                const bool look = r0 + i < nrow && (loc[i] & kEdgeFlag) && root[i] < n;
+                const uint32_t v = L[look ? root[i] : 0u];
+                par[i] = look ? v : root[i];
             }
-            if (flags & kFlip)
-                d.spins[site] ^= static_cast<uint8_t>(x & 1u);
-            if (flags & kWriteLabels)
-                L[site] = x;
+            bool any2 = false;
+#pragma unroll
+            for (uint32_t i = 0; i < kCh; ++i)
+                any2 = any2 || (par[i] != root[i]);
+            if (__ballot(any2))
+            {
+#pragma unroll
+                for (uint32_t i = 0; i < kCh; ++i)
+                {
+                    const bool look = par[i] != root[i] && par[i] < root[i];
+                    const uint32_t v = L[look ? par[i] : 0u];
+                    gpar[i] = look ? v : par[i];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (uint32_t i = 0; i < kCh; ++i)
+                    gpar[i] = par[i];
+            }
+            uint32_t slow = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kCh; ++i)
+            {
+                const uint32_t site = row0 + (r0 + i) * bl.stride;
+                const bool bad = root[i] >= n || par[i] > root[i] || gpar[i] > par[i];
+                const bool resolved = r0 + i < nrow && !bad && gpar[i] == par[i];
+                ok = ok && !(r0 + i < nrow && bad);
+                slow |= (r0 + i < nrow && !bad && gpar[i] != par[i]) ? (1u << i) : 0u;
+                if (resolved)
+                {
+                    if (do_flip)
+                        spins[site] = static_cast<uint8_t>(sp[i] ^ (par[i] & 1u));
+                    if (write_labels)
+                        L[site] = par[i];
+                }
+            }
+            // chains longer than two hops
+            for (uint32_t i = 0; i < kCh; ++i)
+            {
+                if ((slow >> i) & 1u)
+                {
+                    const uint32_t site = row0 + (r0 + i) * bl.stride;
+                    const uint32_t start = SiteOf(bl.base, bl.stride, local[site] & 0x7FFFu);
+                    uint32_t xx = start, p = L[xx];
+                    while (p != xx)
+                    {
+                        if (p > xx)
+                        {
+                            ok = false;
+                            break;
+                        }
+                        xx = p;
+                        p = L[xx];
+                    }
+                    if (p == xx)
+                    {
+                        L[start] = xx;
+                        if (do_flip)
+                            spins[site] = static_cast<uint8_t>(spins[site] ^ (xx & 1u));
+                        if (write_labels)
+                            L[site] = xx;
+                    }
+                }
+            }
         }
-        if ((flags & kAdvance) && tl.lt == 0 && lane == 0)
-            d.state[0] += 2u;
+        if (!ok)
+            (void)__hip_atomic_fetch_or(Glob(d.state) + 3, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((flags & kAdvance) && bl.lb == 0 && threadIdx.x == 0)
+            Glob(d.state)[0] += 2u;
     }
 
     // =============================================================================================
     // Observables: per-tile partial sums of sum s(s_right + s_down) and sum s (lattice.cpp:29-67)
     // =============================================================================================
-    __global__ __launch_bounds__(256) void k_observables(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ tile_lat,
-                                                         const uint32_t total_tiles, const uint32_t nblocks)
+    __global__ __launch_bounds__(256) void k_observables(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                         const uint32_t total_blocks)
     {
-        uint32_t t;
-        if (!TileOfWave(nblocks, total_tiles, t))
+        const uint32_t b = RemapBlock(total_blocks);
+        if (b >= total_blocks)
             return;
+        const LatDesc& d = descs[block_lat[b]];
+        const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
-        const LatDesc& d = descs[tile_lat[t]];
-        const Tile tl = DecodeTile(d, t);
+        const uint32_t w = Uniform(threadIdx.x >> 6);
+        const uint32_t r_begin = w * d.twr;
+        const uint32_t nrow = bl.nwr > r_begin ? min(d.twr, bl.nwr - r_begin) : 0u;
         const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw;
-        const uint32_t x = tl.sx * tw + (lane & (tw - 1u));
+        const uint32_t x = bl.sx * tw + (lane & (tw - 1u));
         int32_t e = 0, m = 0;
-        for (uint32_t r = 0; r < tl.nrow; ++r)
+        for (uint32_t r = 0; r < nrow; ++r)
         {
-            const uint32_t y = (tl.wy0 + r) * rpw + lane / tw;
-            const uint32_t s = d.spins[y * n0 + x];
-            const uint32_t sr = d.spins[y * n0 + ((x + 1u == n0) ? 0u : x + 1u)];
-            const uint32_t sd = d.spins[((y + 1u == n1) ? 0u : y + 1u) * n0 + x];
+            const uint32_t y = (bl.wyb + r_begin + r) * rpw + lane / tw;
+            const uint32_t s = d.spins[y * n0 + x] & 1u;
+            const uint32_t sr = d.spins[y * n0 + ((x + 1u == n0) ? 0u : x + 1u)] & 1u;
+            const uint32_t sd = d.spins[((y + 1u == n1) ? 0u : y + 1u) * n0 + x] & 1u;
             e += 2 * (__popcll(__ballot(s == sr)) + __popcll(__ballot(s == sd))) - 128;
             m += 2 * __popcll(__ballot(s != 0)) - 64;
         }
         if (lane == 0)
         {
-            d.partials[2u * tl.lt] = e;
-            d.partials[2u * tl.lt + 1u] = m;
+            const uint32_t t = kWavesPerBlock * bl.lb + w;
+            d.partials[2u * t] = e;
+            d.partials[2u * t + 1u] = m;
         }
     }
 
@@ -454,7 +806,7 @@ namespace swmi
         __shared__ long long red_e[256], red_m[256];
         const LatDesc& d = descs[blockIdx.x];
         long long e = 0, m = 0;
-        for (uint32_t i = threadIdx.x; i < d.n_tiles; i += 256u)
+        for (uint32_t i = threadIdx.x; i < kWavesPerBlock * d.n_blocks; i += 256u)
         {
             e += d.partials[2u * i];
             m += d.partials[2u * i + 1u];
@@ -498,8 +850,10 @@ namespace swmi
     // =============================================================================================
     // LCG32_State<64>::Init (random/src/lcg32.cpp:31-49) of every group, one thread per group, with the
     // POSIX drand48 recurrence in 48-bit integer arithmetic and IEEE double products.
-    __global__ __launch_bounds__(256) void k_init_rng(uint32_t* __restrict__ rng, uint64_t* __restrict__ masks, uint32_t* __restrict__ rot,
-                                                      const uint32_t n_groups, const uint32_t seed)
+    // The selector of lane i (bits 1-3 of the spin byte of site 64g+i) starts as sel0[i]; run after the
+    // spins are set.
+    __global__ __launch_bounds__(256) void k_init_rng(uint8_t* __restrict__ spins, uint32_t* __restrict__ rng, uint64_t* __restrict__ masks,
+                                                      uint32_t* __restrict__ rot, const uint32_t n_groups, const uint32_t seed)
     {
         const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
         if (g >= n_groups)
@@ -519,15 +873,21 @@ namespace swmi
         xs = (kA * xs + kC) & kMask;
         const double dr = static_cast<double>(xs) * 0x1p-48;
         uint32_t a, c;
-        LcgParams(static_cast<uint32_t>((m0 & 1ull) | ((m1 & 1ull) << 1) | ((m2 & 1ull) << 2)), a, c);
+        ParamsOf(static_cast<uint32_t>((m0 & 1ull) | ((m1 & 1ull) << 1) | ((m2 & 1ull) << 2)), a, c);
         uint32_t st = a * (static_cast<uint32_t>(__dmul_rn(4026531839.0, dr)) + 1u) + c;
         uint32_t* out = rng + static_cast<size_t>(g) * 64u;
+        uint8_t* sp = spins + static_cast<size_t>(g) * 64u;
         out[0] = st;
-        for (uint32_t i = 1; i < 64u; ++i)
+        for (uint32_t i = 0; i < 64u; ++i)
         {
-            LcgParams(static_cast<uint32_t>(((m0 >> i) & 1ull) | (((m1 >> i) & 1ull) << 1) | (((m2 >> i) & 1ull) << 2)), a, c);
-            st = a * st + c;
-            out[i] = st;
+            const uint32_t k = static_cast<uint32_t>(((m0 >> i) & 1ull) | (((m1 >> i) & 1ull) << 1) | (((m2 >> i) & 1ull) << 2));
+            sp[i] = static_cast<uint8_t>((sp[i] & 1u) | (k << 1));
+            if (i > 0)
+            {
+                ParamsOf(k, a, c);
+                st = a * st + c;
+                out[i] = st;
+            }
         }
         masks[3u * g] = m0;
         masks[3u * g + 1u] = m1;
@@ -544,7 +904,7 @@ namespace swmi
         if (first >= n)
             return;
         const uint64_t kMask = (1ull << 48) - 1ull;
-        // (ja, jc) = affine map x -> A x + C raised to the power first
+        // (ja, jc) = affine map x -> A x + C raised to the power `first`
         uint64_t ja = 1, jc = 0, ba = 0x5DEECE66Dull, bc = 0xBull;
         for (uint64_t e = first; e; e >>= 1)
         {
@@ -566,6 +926,14 @@ namespace swmi
         }
     }
 
+    // Upload: replace bit 0 of every spin byte, keep the lcg32 selector bits.
+    __global__ __launch_bounds__(256) void k_set_spins(uint8_t* __restrict__ spins, const uint8_t* __restrict__ src, const uint64_t n)
+    {
+        const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (i < n)
+            spins[i] = static_cast<uint8_t>((spins[i] & 0xFEu) | (src[i] & 1u));
+    }
+
     __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ labels, const uint64_t n)
     {
         const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -583,40 +951,46 @@ namespace swmi
     // =============================================================================================
     // launch wrappers
     // =============================================================================================
-    static inline uint32_t Blocks(const uint32_t total_tiles) { return (total_tiles + kWavesPerBlock - 1) / kWavesPerBlock; }
-    static inline uint32_t Grid(const uint32_t nblocks) { return ((nblocks + kNumXcd - 1) / kNumXcd) * kNumXcd; }
+    static inline uint32_t XcdGrid(const uint32_t units) { return ((units + kNumXcd - 1) / kNumXcd) * kNumXcd; }
 
-    void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_tile_lat, const uint32_t total_tiles, const uint32_t max_twr,
-                         hipStream_t stream)
+    template <bool TW64>
+    static void LaunchBondLabelT(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks,
+                                 const uint32_t max_twr, hipStream_t stream)
     {
-        const uint32_t nb = Blocks(total_tiles);
+        const uint32_t g = XcdGrid(total_blocks);
         if (max_twr <= 8)
-            k_bond_label<8><<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb);
+            k_bond_label<8, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
         else if (max_twr <= 16)
-            k_bond_label<16><<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb);
+            k_bond_label<16, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
         else if (max_twr <= 32)
-            k_bond_label<32><<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb);
+            k_bond_label<32, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
         else
-            k_bond_label<64><<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb);
+            k_bond_label<64, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
-    void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_tile_lat, const uint32_t total_tiles, hipStream_t stream)
+    void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
+                         const bool all_tw64, hipStream_t stream)
     {
-        const uint32_t nb = Blocks(total_tiles);
-        k_merge<<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb);
+        if (all_tw64)
+            LaunchBondLabelT<true>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+        else
+            LaunchBondLabelT<false>(d_desc, d_block_lat, total_blocks, max_twr, stream);
     }
 
-    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_tile_lat, const uint32_t total_tiles, const uint32_t flags,
+    void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
+    {
+        k_merge<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+    }
+
+    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t flags,
                     hipStream_t stream)
     {
-        const uint32_t nb = Blocks(total_tiles);
-        k_flip<<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb, flags);
+        k_flip<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks, flags);
     }
 
-    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_tile_lat, const uint32_t total_tiles, hipStream_t stream)
+    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
     {
-        const uint32_t nb = Blocks(total_tiles);
-        k_observables<<<Grid(nb), 256, 0, stream>>>(d_desc, d_tile_lat, total_tiles, nb);
+        k_observables<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
     void LaunchFinalizeSamples(const LatDesc* d_desc, const uint32_t n_lattices, hipStream_t stream)
@@ -624,10 +998,15 @@ namespace swmi
         k_finalize_samples<<<n_lattices, 256, 0, stream>>>(d_desc);
     }
 
-    void LaunchInitRng(uint32_t* rng, uint64_t* masks, uint32_t* rot, const uint32_t n_groups, const uint32_t seed,
+    void LaunchInitRng(uint8_t* spins, uint32_t* rng, uint64_t* masks, uint32_t* rot, const uint32_t n_groups, const uint32_t seed,
                        hipStream_t stream)
     {
-        k_init_rng<<<(n_groups + 255) / 256, 256, 0, stream>>>(rng, masks, rot, n_groups, seed);
+        k_init_rng<<<(n_groups + 255) / 256, 256, 0, stream>>>(spins, rng, masks, rot, n_groups, seed);
+    }
+
+    void LaunchSetSpins(uint8_t* spins, const uint8_t* src, const uint64_t n, hipStream_t stream)
+    {
+        k_set_spins<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, stream>>>(spins, src, n);
     }
 
     void LaunchHotStart(uint8_t* spins, const uint64_t n, const uint32_t spin_seed, hipStream_t stream)

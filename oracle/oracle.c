@@ -7,15 +7,16 @@
  *
  * Pinning (what checks each function, so that a plausible mistake fails one of them):
  *   or_srand48/or_drand48   glibc drand48/srand48 called through ctypes (tests/test_oracle.py)
- *   or_lcg32_*              golden streams from the compiled reference LCG32_State<16>/<64>
- *                           (tests/golden/lcg32_*.json, written by oracle/_ref/ref_golden)
- *   or_hot_start            golden spin dump of the compiled reference Lattice<2> constructor
- *   or_p_add/or_threshold   reference expression compiled with the reference flags (golden) and an
+ *   or_lcg32_*              golden streams of the compiled reference LCG32_State<16>/<64> and
+ *                           LCG32<CPU>::NextInteger calls (tests/golden/lcg32_streams.npz,
+ *                           lcg32_buffered.json, written by tests/golden/make_golden.py)
+ *   or_hot_start            sweep-0 snapshots of the compiled reference Lattice<2> (refcpu.npz)
+ *   or_p_add/or_threshold   reference expression compiled with the reference flags (padd.json) and an
  *                           exhaustive 2^32 check of the threshold against the float compare
  *   or_label                brute-force BFS / scipy connected components (min site index per cluster)
  *                           and the compiled reference trajectories
  *   or_refcpu_*             bit-exact spin trajectories of the compiled reference CPU path at
- *                           NUM_THREADS in {1,2,3,4,32} (tests/golden/refcpu_*.npz)
+ *                           NUM_THREADS in {1,2,3,4,5,32} (tests/golden/refcpu.npz)
  *   or_observables          brute-force numpy sums
  *   or_ps_*                 composition of the pinned pieces above, plus the rotation invariant
  *                           checked against a full or_lcg32 per group
@@ -183,6 +184,14 @@ uint64_t or_threshold(float p_add)
             hi = mid;
     }
     return lo;
+}
+
+uint64_t or_threshold_mismatches(float p_add, uint64_t threshold)
+{
+    uint64_t bad = 0;
+    for (uint64_t u = 0; u < (1ULL << 32); ++u)
+        bad += (uint64_t)(bond_draw((uint32_t)u, p_add) != (u < threshold));
+    return bad;
 }
 
 /* ------------------------------------------------------------------------------------------------

@@ -54,6 +54,8 @@ void or_hot_start(uint8_t* spins, size_t n, uint32_t seed);
 float or_p_add(float temperature);
 /* Smallest u with (2^-32f * (float)u) >= p_add, so that "r < p_add" <=> "u < threshold". */
 uint64_t or_threshold(float p_add);
+/* Number of draws u in [0, 2^32) for which (u < threshold) differs from the reference's float compare. */
+uint64_t or_threshold_mismatches(float p_add, uint64_t threshold);
 
 /* ---- labeller, flip, observables ----
  * Bond bytes: bit 0 = bond to (x+1 mod N0, y), bit 1 = bond to (x, y+1 mod N1). Site index y*N0+x. */

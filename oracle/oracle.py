@@ -52,6 +52,8 @@ def lib() -> ctypes.CDLL:
     L.or_p_add.restype = ctypes.c_float
     L.or_threshold.argtypes = [ctypes.c_float]
     L.or_threshold.restype = u64
+    L.or_threshold_mismatches.argtypes = [ctypes.c_float, u64]
+    L.or_threshold_mismatches.restype = u64
     L.or_label.argtypes = [u32, u32, u32, u32, vp, vp]
     L.or_flip.argtypes = [ctypes.c_size_t, vp, vp]
     L.or_observables.argtypes = [u32, u32, vp, P(ctypes.c_int64), P(ctypes.c_int64)]

@@ -1,0 +1,234 @@
+"""The oracle (CPU restatement of the reference path) against the reference's golden vectors and against
+what the mathematics fixes. CPU only.
+
+Each check is chosen so that a plausible mistake in the restatement fails it: a wrong drand48 scaling,
+a lane rotation off by one, a skipped state[0], a bond decided on the wrong draw, a dropped merge, a
+flip of the wrong parity.
+"""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
+
+from conftest import GOLDEN, ROOT
+
+TEMPS_REF = ["2.0", "2.2691853", "2.6"]
+
+
+# ------------------------------------------------------------------------------------------------
+# drand48: restated from POSIX, checked against glibc's own srand48/drand48 (library routine)
+# ------------------------------------------------------------------------------------------------
+def test_drand48_matches_glibc(oracle):
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    for seed in (0, 1, 2, 77, 123456789, 2**32 - 1):
+        libc.srand48(seed)
+        x = ctypes.c_uint64()
+        oracle.lib().or_srand48(ctypes.byref(x), seed & 0xFFFFFFFF)
+        for _ in range(1000):
+            assert oracle.lib().or_drand48(ctypes.byref(x)) == libc.drand48()
+
+
+def test_hot_start_matches_reference_lattice(oracle):
+    """Lattice<2> constructor (lattice.cpp:15-27): the sweep-0 snapshots of the reference runs."""
+    g = np.load(GOLDEN / "refcpu.npz")
+    meta = json.loads(bytes(g["meta"]).decode())
+    for m in meta:
+        spins0 = g[m["name"] + "_spins"][0]
+        np.testing.assert_array_equal(oracle.hot_start(m["n0"] * m["n1"], 1), spins0)
+
+
+# ------------------------------------------------------------------------------------------------
+# lcg32
+# ------------------------------------------------------------------------------------------------
+def test_lcg32_streams_match_reference(oracle):
+    g = np.load(GOLDEN / "lcg32_streams.npz")
+    for key in g.files:
+        w = int(key.split("_")[0][1:])
+        seed = int(key.split("seed")[1])
+        ref = g[key]
+        got = np.array(oracle.lcg32_states(w, seed, ref.shape[0] - 1), dtype=np.uint32)
+        np.testing.assert_array_equal(got, ref, err_msg=key)
+
+
+def test_lcg32_buffered_matches_reference(oracle):
+    for case in json.loads((GOLDEN / "lcg32_buffered.json").read_text()):
+        got = oracle.lcg32_buffered(case["w"], case["seed"], case["sizes"])
+        assert got == case["calls"]
+
+
+def test_lcg32_shuffle_invariant(oracle):
+    """Between shuffles every lane is a plain LCG with its own (a, c); every 15th update rotates (a, c)."""
+    s = oracle.Lcg32()
+    oracle.lib().or_lcg32_init(ctypes.byref(s), 64, 5)
+    for it in range(1, 60):
+        before = list(s.state)
+        a_before = list(s.a)
+        oracle.lib().or_lcg32_update(ctypes.byref(s))
+        if it % 15:
+            assert list(s.a) == a_before
+            for i in range(64):
+                assert s.state[i] == (a_before[i] * before[i] + s.c[i]) & 0xFFFFFFFF
+        else:
+            shift = (before[it & 63] + (0 if it & 1 else 1)) & 63
+            assert list(s.a) == [a_before[(i + shift) & 63] for i in range(64)]
+
+
+def test_per_site_layout_equals_full_lcg32_state(oracle):
+    """The engine's compact group state (selector planes + rotation) reproduces LCG32_State<64>."""
+    n_groups = 5
+    seed = 31
+    states = np.empty(64 * n_groups, dtype=np.uint32)
+    masks = np.empty(3 * n_groups, dtype=np.uint64)
+    rot = np.empty(n_groups, dtype=np.uint32)
+    oracle.lib().or_ps_init(n_groups, seed, states.ctypes.data, masks.ctypes.data, rot.ctypes.data)
+    full = []
+    for g in range(n_groups):
+        s = oracle.Lcg32()
+        oracle.lib().or_lcg32_init(ctypes.byref(s), 64, oracle.lib().or_group_seed(seed, g))
+        full.append(s)
+    for it in range(200):
+        oracle.lib().or_ps_rng_update(n_groups, states.ctypes.data, masks.ctypes.data, rot.ctypes.data, it)
+        for g in range(n_groups):
+            oracle.lib().or_lcg32_update(ctypes.byref(full[g]))
+            np.testing.assert_array_equal(states[64 * g:64 * g + 64], np.array(full[g].state[:], dtype=np.uint32))
+    assert oracle.lib().or_group_seed(0, 0) == 1 and oracle.lib().or_group_seed(0, 9) == 10
+
+
+# ------------------------------------------------------------------------------------------------
+# bond probability
+# ------------------------------------------------------------------------------------------------
+def test_p_add_bits_match_reference(oracle):
+    ref = json.loads((GOLDEN / "padd.json").read_text())
+    for t, bits in ref.items():
+        got = np.array([oracle.p_add(float(np.float32(float(t))))], dtype=np.float32).view(np.uint32)[0]
+        assert int(got) == bits, t
+    # bit patterns quoted in the survey of this path (glibc 2.35 expf)
+    assert ref["2.2691853"] == 0x3F15F61A and ref["2.0"] == 0x3F21D2A7 and ref["2.6"] == 0x3F0960A0
+
+
+@pytest.mark.parametrize("t", TEMPS_REF)
+def test_threshold_exhaustive(oracle, t):
+    """u < threshold equals 2^-32f * (float)u < p_add for all 2^32 draws."""
+    p = oracle.lib().or_p_add(float(t))
+    thr = oracle.lib().or_threshold(p)
+    assert oracle.lib().or_threshold_mismatches(p, thr) == 0
+    known = {"2.2691853": 2515933568, "2.0": 2714936961, "2.6": 2304810880}
+    assert thr == known[t]
+
+
+# ------------------------------------------------------------------------------------------------
+# labeller: the restated reference labeller against connected components found independently
+# ------------------------------------------------------------------------------------------------
+def components_min_label(n0, n1, bonds):
+    idx = np.arange(n0 * n1).reshape(n1, n0)
+    right = np.roll(idx, -1, axis=1).ravel()
+    down = np.roll(idx, -1, axis=0).ravel()
+    bx = (bonds & 1).astype(bool)
+    by = (bonds & 2).astype(bool)
+    rows = np.concatenate([np.arange(n0 * n1)[bx], np.arange(n0 * n1)[by]])
+    cols = np.concatenate([right[bx], down[by]])
+    g = coo_matrix((np.ones(rows.size), (rows, cols)), shape=(n0 * n1, n0 * n1))
+    _, comp = connected_components(g, directed=False)
+    mins = np.full(comp.max() + 1, n0 * n1, dtype=np.int64)
+    np.minimum.at(mins, comp, np.arange(n0 * n1))
+    return mins[comp].astype(np.uint32)
+
+
+@pytest.mark.parametrize("n0,n1", [(2, 2), (4, 6), (16, 8), (32, 32), (48, 34), (64, 64), (130, 18), (2, 64)])
+@pytest.mark.parametrize("tile", [(16, 8), (4, 4), (64, 2)])
+def test_label_equals_connected_components(oracle, n0, n1, tile):
+    rng = np.random.default_rng(n0 * 7 + n1 + tile[0])
+    for density in (0.0, 0.2, 0.5, 0.8, 1.0):
+        bonds = ((rng.random(n0 * n1) < density).astype(np.uint8) | ((rng.random(n0 * n1) < density).astype(np.uint8) << 1))
+        np.testing.assert_array_equal(oracle.label(n0, n1, bonds, tile), components_min_label(n0, n1, bonds))
+
+
+def test_observables_bruteforce(oracle):
+    rng = np.random.default_rng(3)
+    for n0, n1 in [(2, 2), (8, 6), (64, 32)]:
+        s = rng.integers(0, 2, n0 * n1).astype(np.uint8)
+        v = 2 * s.reshape(n1, n0).astype(np.int64) - 1
+        e = int((v * (np.roll(v, -1, 1) + np.roll(v, -1, 0))).sum())
+        assert oracle.observables(n0, n1, s) == (e, int(v.sum()))
+
+
+# ------------------------------------------------------------------------------------------------
+# the whole reference CPU path, bit-exact against the compiled reference's trajectories
+# ------------------------------------------------------------------------------------------------
+def test_reference_cpu_trajectories(oracle):
+    g = np.load(GOLDEN / "refcpu.npz")
+    meta = json.loads(bytes(g["meta"]).decode())
+    for m in meta:
+        n0, n1, T = m["n0"], m["n1"], float(np.float32(float(m["temperature"])))
+        spins = oracle.hot_start(n0 * n1, 1)
+        r = oracle.RefCpu(n0, n1, 16, m["num_threads"])
+        done = 0
+        for i, k in enumerate(m["sweeps"]):
+            while done < k:
+                r.sweep(spins, T)
+                done += 1
+            np.testing.assert_array_equal(spins, g[m["name"] + "_spins"][i], err_msg=f"{m['name']} after {k} sweeps")
+            assert oracle.observables(n0, n1, spins) == tuple(int(v) for v in g[m["name"] + "_sums"][i])
+
+
+def test_parity_flip_freezes_site_zero(oracle):
+    """Site 0 is always the minimum of its cluster (even root), so the reference rule never flips it."""
+    p = oracle.PerSite(64, 64, 2.2691853, seed=4)
+    s0 = p.spins[0]
+    for _ in range(50):
+        p.sweep()
+        assert p.spins[0] == s0
+
+
+@pytest.mark.skipif(not (ROOT / "oracle" / "_ref" / "ref_sw").exists(), reason="reference binaries are built only where the reference sources exist")
+def test_reference_cpu_fresh_run(oracle, tmp_path):
+    """Directly against a fresh run of the compiled reference (beyond the committed fixtures)."""
+    n0, n1, T, nt, ks = 80, 66, "2.35", 6, [1, 9]
+    out = tmp_path / "d.bin"
+    subprocess.run([str(ROOT / "oracle" / "_ref" / "ref_sw"), "dump", str(n0), str(n1), T, str(out)] + [str(k) for k in ks],
+                   check=True, capture_output=True, env=dict(os.environ, NUM_THREADS=str(nt)))
+    raw = np.fromfile(out, dtype=np.uint8)
+    spins = oracle.hot_start(n0 * n1, 1)
+    r = oracle.RefCpu(n0, n1, 16, nt)
+    done = 0
+    for i, k in enumerate(ks):
+        while done < k:
+            r.sweep(spins, float(np.float32(float(T))))
+            done += 1
+        np.testing.assert_array_equal(spins, raw[i * (n0 * n1 + 16):i * (n0 * n1 + 16) + n0 * n1])
+
+
+# ------------------------------------------------------------------------------------------------
+# exact finite-lattice values (the statistical oracle)
+# ------------------------------------------------------------------------------------------------
+def test_exact_values_fixture():
+    ex = json.loads((GOLDEN / "exact_values.json").read_text())
+    # README.md:98 quotes U/N at the critical coupling on 128^2
+    assert abs(ex["critical_128"]["U_per_spin"] - (-1.419076272084983)) < 1e-12
+    vals = {(v["L"], v["T"]): v for v in ex["values"]}
+    # the values quoted for this path at K = 1/float(T)
+    assert abs(vals[(128, "2.2691853")]["U_per_spin"] - (-1.419076296360968)) < 1e-12
+    assert abs(vals[(1024, "2.2691853")]["U_per_spin"] - (-1.414821447346456)) < 1e-12
+    assert abs(vals[(4096, "2.2691853")]["C_per_spin"] - 4.25155843809) < 1e-8
+    # away from T_c the finite-size effects vanish: every L gives the same U/N at T = 2.0
+    assert abs(vals[(128, "2.0")]["U_per_spin"] - vals[(8192, "2.0")]["U_per_spin"]) < 1e-12
+
+
+@pytest.mark.skipif(not (ROOT / "oracle" / "_ref" / "ff_exact").exists(), reason="reference binaries are built only where the reference sources exist")
+def test_exact_values_match_reference_tool():
+    """The reference's long-double closed-form tool agrees with the numerical-derivative route."""
+    ex = json.loads((GOLDEN / "exact_values.json").read_text())
+    for v in ex["values"]:
+        if v["L"] > 128:
+            continue
+        out = subprocess.run([str(ROOT / "oracle" / "_ref" / "ff_exact"), repr(v["K"]), str(v["L"]), str(v["L"])],
+                             check=True, capture_output=True, text=True).stdout.split()
+        assert abs(float(out[1]) - v["U_per_spin"]) < 1e-9
+        assert abs(float(out[2]) - v["C_per_spin"]) < 1e-6
