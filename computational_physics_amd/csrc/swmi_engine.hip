@@ -114,7 +114,7 @@ namespace
         swmi_lattice_config cfg{};
         uint64_t n = 0;
         uint32_t n_groups = 0;
-        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0;
+        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0;
         float p_add = 0.0f;
         uint64_t threshold = 0;
         DeviceBuffer<uint8_t> spins;
@@ -126,6 +126,8 @@ namespace
         DeviceBuffer<uint64_t> bbits;
         DeviceBuffer<uint32_t> state;
         DeviceBuffer<int32_t> partials;
+        DeviceBuffer<uint16_t> edge_list;
+        DeviceBuffer<uint32_t> edge_count;
         DeviceBuffer<uint8_t> bond_dump;
         DeviceBuffer<uint8_t> bond_in;
     };
@@ -176,6 +178,9 @@ struct swmi_engine
         d.bond_dump = l.bond_dump.ptr;
         d.bond_in = nullptr;
         d.partials = l.partials.ptr;
+        d.edge_list = l.edge_list.ptr;
+        d.edge_count = l.edge_count.ptr;
+        d.edge_cap = l.edge_cap;
         d.n0 = l.cfg.n0;
         d.n1 = l.cfg.n1;
         d.tw = l.tw;
@@ -227,7 +232,7 @@ struct swmi_engine
         {
             swmi::LaunchBondLabel(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, all_tw64, stream);
             swmi::LaunchMerge(d_desc.ptr, d_block_lat.ptr, total_blocks, stream);
-            swmi::LaunchFlip(d_desc.ptr, d_block_lat.ptr, total_blocks, swmi::kFlip | swmi::kAdvance, stream);
+            swmi::LaunchFlip(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, swmi::kFlip | swmi::kAdvance, stream);
         }
         LaunchCheck("sweep launch");
     }
@@ -292,6 +297,8 @@ namespace
         l.twr = std::max<uint32_t>(twr, 1u);
         l.blocks_y = (l.nwr_y + swmi::kWavesPerBlock * l.twr - 1) / (swmi::kWavesPerBlock * l.twr);
         l.n_blocks = l.tiles_x * l.blocks_y;
+        // distinct clusters reaching a block's boundary <= boundary sites (2 columns + 2 rows)
+        l.edge_cap = std::min<uint32_t>(l.twr * 64u, 2u * l.tw + 2u * swmi::kWavesPerBlock * l.twr * l.rpw);
         l.p_add = BondProbability(c.temperature);
         l.threshold = BondThreshold(l.p_add);
 
@@ -305,6 +312,9 @@ namespace
         const size_t state_words = swmi::kLatStateWords + 4ull * c.max_samples;
         l.state.allocate(state_words);
         l.partials.allocate(2ull * swmi::kWavesPerBlock * l.n_blocks);
+        l.edge_list.allocate(static_cast<size_t>(swmi::kWavesPerBlock) * l.n_blocks * l.edge_cap);
+        l.edge_count.allocate(static_cast<size_t>(swmi::kWavesPerBlock) * l.n_blocks);
+        Check(hipMemsetAsync(l.edge_count.ptr, 0, l.edge_count.count * 4, stream), "edge list init");
         Check(hipMemsetAsync(l.state.ptr, 0, state_words * 4, stream), "state init");
         Check(hipMemcpyAsync(reinterpret_cast<uint8_t*>(l.state.ptr) + 8, &c.max_samples, 4, hipMemcpyHostToDevice, stream),
               "state init");
@@ -618,7 +628,7 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
         engine->OneLattice(l, l.bond_in.ptr);
         swmi::LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, l.tw == 64, engine->stream);
         swmi::LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, engine->stream);
-        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, swmi::kWriteLabels, engine->stream);
+        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, swmi::kWriteLabels, engine->stream);
         engine->LaunchCheck("label launch");
         Check(hipStreamSynchronize(engine->stream), "label");
     });
@@ -631,7 +641,7 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = engine->At(lattice);
         engine->OneLattice(l, nullptr);
-        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, swmi::kWriteLabels, engine->stream);
+        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, swmi::kWriteLabels, engine->stream);
         engine->LaunchCheck("resolve launch");
         Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
         Check(hipStreamSynchronize(engine->stream), "label download");
@@ -683,7 +693,7 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
             Check(hipEventRecord(ev[1], engine->stream), "event");
             swmi::LaunchMerge(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->stream);
             Check(hipEventRecord(ev[2], engine->stream), "event");
-            swmi::LaunchFlip(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, swmi::kFlip | swmi::kAdvance, engine->stream);
+            swmi::LaunchFlip(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->max_twr, swmi::kFlip | swmi::kAdvance, engine->stream);
             Check(hipEventRecord(ev[3], engine->stream), "event");
             Check(hipEventSynchronize(ev[3]), "event sync");
             for (int k = 0; k < 3; ++k)

@@ -9,6 +9,9 @@
 //   parent   uint32[n]       global union-find over the roots of edge-reaching clusters (other entries
 //                            unused); after a resolve pass, site -> canonical (minimum) site index
 //   bbits    uint64[5*blocks] block-edge bond bits: down edge, then right edge (64 rows per word)
+//   edge_list  uint16[4*blocks*edge_cap]  per tile: block-local indices of the roots of its clusters
+//                            that reach the block edge (the only roots the resolve pass must chase)
+//   edge_count uint32[4*blocks]           entries of each tile's list
 //
 // Tiling. A "wave row" is 64 consecutive row-major sites, which is exactly one lcg32 group:
 //   n0 >= 64 (n0 % 64 == 0): a wave row is 64 columns of one lattice row;
@@ -43,6 +46,9 @@ namespace swmi
         uint8_t* bond_dump;      // per-site bond bytes of the last sweep (nullptr: not recorded)
         const uint8_t* bond_in;  // injected bond bytes (label-only mode), nullptr otherwise
         int32_t* partials;       // 2 int32 (E, M) per tile for observables
+        uint16_t* edge_list;     // kWavesPerBlock lists of edge_cap entries per block
+        uint32_t* edge_count;    // kWavesPerBlock counts per block
+        uint32_t edge_cap;       // capacity of one tile's list: min(tile sites, block boundary sites)
         uint32_t n0, n1;
         uint32_t tw;             // tile width in sites: min(n0, 64)
         uint32_t rpw;            // lattice rows per wave row: 64 / tw
@@ -82,8 +88,8 @@ namespace swmi
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
                          bool all_tw64, hipStream_t stream);
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
-    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t flags,
-                    hipStream_t stream);
+    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
+                    uint32_t flags, hipStream_t stream);
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
     void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, hipStream_t stream);
     void LaunchInitRng(uint8_t* spins, uint32_t* rng, uint64_t* masks, uint32_t* rot, uint32_t n_groups, uint32_t seed,

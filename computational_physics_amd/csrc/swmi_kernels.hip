@@ -12,8 +12,9 @@
 // per-group data except on the sweeps that rotate (one in 7.5).
 //   k_merge       per block: unions across block edges by atomicMin on the uint32 parent array in HBM,
 //                 over the roots of edge-reaching clusters only.
-//   k_flip        per block: a site's root is its block-local root, or (edge clusters) the chase of the
-//                 global parent array from it; spin ^= root & 1. Advances the lcg32 iteration count.
+//   k_flip        per block: the listed edge-cluster roots are chased in the global parent array (one
+//                 chase per root, kept in LDS); then every site's label is its block-local root or that
+//                 LDS entry, and spin ^= label & 1. Advances the lcg32 iteration count.
 // Reference behaviour: ising/src/swendsen_wang_amd_gpu.cpp:78-467 (four kernels per sweep) and the CPU
 // labeller ising/src/swendsen_wang_cpu.cpp:13-205; the kernels here are written for wave64 CDNA4.
 #include <hip/hip_runtime.h>
@@ -35,6 +36,10 @@ namespace swmi
     {
         return (gptr<T>)(p);
     }
+
+    // plain vector types (HIP's uint2/uint4 classes cannot be assigned through address-space-1 pointers)
+    using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
+    using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
 
     // bit `lane` of a wave-uniform mask, as a per-lane predicate (v_cndmask on the SGPR pair)
     __device__ __forceinline__ bool LaneBit(const uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
@@ -552,6 +557,11 @@ namespace swmi
         }
         __syncthreads();
 
+        // labels out; the roots of edge-reaching clusters start their global union-find entry and are
+        // listed (per tile) for the resolve pass
+        const uint32_t cap = d.edge_cap;
+        const gptr<uint16_t> elist = Glob(d.edge_list) + (kWavesPerBlock * bl.lb + w) * cap;
+        uint32_t nedge = 0;
         for (uint32_t r = 0; r < nrow; ++r)
         {
             const uint32_t q = (r_begin + r) * 64u + lane;
@@ -559,8 +569,16 @@ namespace swmi
             const uint32_t root = LdsLoad(&P[q]);
             const bool flagged = ((LdsLoad(&F[root >> 5]) >> (root & 31u)) & 1u) != 0u;
             local[site] = static_cast<uint16_t>(root | (flagged ? kEdgeFlag : 0u));
-            if (flagged && root == q)
+            const uint64_t K = __ballot(flagged && root == q);
+            if (LaneBit(K))
+            {
                 parent[site] = site;
+                const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
+                if (pos < cap)
+                    elist[pos] = static_cast<uint16_t>(q);
+            }
+            nedge += static_cast<uint32_t>(__popcll(K));
         }
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         if (lane == 0)
@@ -568,6 +586,9 @@ namespace swmi
             if (last_wave)
                 bb[0] = down_bits;
             bb[1u + w] = right_bits;
+            Glob(d.edge_count)[kWavesPerBlock * bl.lb + w] = min(nedge, cap);
+            if (nedge > cap)
+                (void)__hip_atomic_fetch_or(Glob(d.state) + 3, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 
@@ -640,123 +661,124 @@ namespace swmi
     // k_flip: resolve + flip (reference ResolveLabels_Kernel / FlipClusters_Kernel,
     // swendsen_wang_amd_gpu.cpp:403-467)
     // =============================================================================================
-    // Per chunk of kCh rows: all block-local labels and spins are fetched, then the parent of every edge
-    // root, then the grandparent; a root whose chain is longer (rare: the merge points every start node
-    // at its root) is finished in a slow pass.
+    // Two phases per block. (1) The roots of the block's edge-reaching clusters (the tile lists written
+    // by k_bond_label) are chased in the global parent array, one root per thread; the canonical label
+    // found is kept in LDS at the root's block-local index. (2) The block is streamed four sites per
+    // lane: a site's label is its block-local root mapped to a site index or, for an edge cluster, the
+    // LDS entry of that root; the spin flips by the label's parity. The first chunk of phase 2 is
+    // loaded before phase 1 so that its memory latency overlaps the chases.
     __global__ __launch_bounds__(256) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                   const uint32_t total_blocks, const uint32_t flags)
     {
-        // loads of a chunk (labels, spins, parents, grandparents) all precede its stores: the memory
-        // counter a load waits on also counts earlier stores
-        constexpr uint32_t kCh = 16;
+        constexpr uint32_t kPass = 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
+        constexpr uint32_t kCh = 4;      // passes loaded together
+        extern __shared__ uint32_t R[];  // kWavesPerBlock * twr * 64 entries
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
         const LatDesc& d = descs[block_lat[b]];
         const Block bl = DecodeBlock(d, b);
-        const uint32_t lane = threadIdx.x & 63u;
-        const uint32_t w = Uniform(threadIdx.x >> 6);
-        const uint32_t r_begin = w * d.twr;
-        const uint32_t nrow = bl.nwr > r_begin ? min(d.twr, bl.nwr - r_begin) : 0u;
         const gptr<uint32_t> L = Glob(d.parent);
         const gptr<uint16_t> local = Glob(d.local);
         const gptr<uint8_t> spins = Glob(d.spins);
-        const uint32_t n = d.n0 * d.n1;
+        const uint32_t qmax = bl.nwr * 64u;
         const bool do_flip = (flags & kFlip) != 0u;
         const bool write_labels = (flags & kWriteLabels) != 0u;
-        const uint32_t row0 = bl.base + r_begin * bl.stride + lane;
+        const uint32_t sub = threadIdx.x >> 4;
+        const uint32_t col = (threadIdx.x & 15u) * 4u;
         bool ok = true;
 
-        for (uint32_t r0 = 0; r0 < nrow; r0 += kCh)
+        u32x2 lv[kCh];
+        uint32_t sv[kCh];
+        const auto load = [&](const uint32_t r0) {
+#pragma unroll
+            for (uint32_t k = 0; k < kCh; ++k)
+            {
+                const uint32_t r = r0 + sub + k * kPass;
+                lv[k] = u32x2{0u, 0u};
+                sv[k] = 0u;
+                if (r < bl.nwr)
+                {
+                    const uint32_t site = bl.base + r * bl.stride + col;
+                    lv[k] = *(gptr<const u32x2>)(local + site);
+                    if (do_flip)
+                        sv[k] = *(gptr<const uint32_t>)(spins + site);
+                }
+            }
+        };
+        load(0u);
+
+        // phase 1: canonical labels of the edge-reaching roots
         {
-            uint32_t loc[kCh], sp[kCh], root[kCh], par[kCh], gpar[kCh];
-#pragma unroll
-            for (uint32_t i = 0; i < kCh; ++i)
+            const uint32_t cap = d.edge_cap;
+            const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
+            const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
+            const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
+            const gptr<uint16_t> elist = Glob(d.edge_list) + kWavesPerBlock * bl.lb * cap;
+            for (uint32_t i = threadIdx.x; i < total; i += 256u)
             {
-                loc[i] = sp[i] = 0;
-                if (r0 + i < nrow)
+                const uint32_t j = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
+                const uint32_t first = j == 0u ? 0u : (j == 1u ? e1 : (j == 2u ? e2 : e3));
+                const uint32_t q = elist[j * cap + (i - first)];
+                if (q >= qmax)
                 {
-                    const uint32_t site = row0 + (r0 + i) * bl.stride;
-                    loc[i] = local[site];
-                    if (do_flip)
-                        sp[i] = spins[site];
+                    ok = false;
+                    continue;
                 }
-            }
-            // loads are unconditional (lanes without a lookup read entry 0) so that the compiler keeps
-            // them as plain global loads
-#pragma unroll
-            for (uint32_t i = 0; i < kCh; ++i)
-            {
-                root[i] = SiteOf(bl.base, bl.stride, loc[i] & 0x7FFFu);
-                const bool look = r0 + i < nrow && (loc[i] & kEdgeFlag) && root[i] < n;
-                const uint32_t v = L[look ? root[i] : 0u];
-                par[i] = look ? v : root[i];
-            }
-            bool any2 = false;
-#pragma unroll
-            for (uint32_t i = 0; i < kCh; ++i)
-                any2 = any2 || (par[i] != root[i]);
-            if (__ballot(any2))
-            {
-#pragma unroll
-                for (uint32_t i = 0; i < kCh; ++i)
+                uint32_t x = SiteOf(bl.base, bl.stride, q);
+                uint32_t p = CachedLoad(&L[x]);
+                while (p != x)
                 {
-                    const bool look = par[i] != root[i] && par[i] < root[i];
-                    const uint32_t v = L[look ? par[i] : 0u];
-                    gpar[i] = look ? v : par[i];
-                }
-            }
-            else
-            {
-#pragma unroll
-                for (uint32_t i = 0; i < kCh; ++i)
-                    gpar[i] = par[i];
-            }
-            uint32_t slow = 0;
-#pragma unroll
-            for (uint32_t i = 0; i < kCh; ++i)
-            {
-                const uint32_t site = row0 + (r0 + i) * bl.stride;
-                const bool bad = root[i] >= n || par[i] > root[i] || gpar[i] > par[i];
-                const bool resolved = r0 + i < nrow && !bad && gpar[i] == par[i];
-                ok = ok && !(r0 + i < nrow && bad);
-                slow |= (r0 + i < nrow && !bad && gpar[i] != par[i]) ? (1u << i) : 0u;
-                if (resolved)
-                {
-                    if (do_flip)
-                        spins[site] = static_cast<uint8_t>(sp[i] ^ (par[i] & 1u));
-                    if (write_labels)
-                        L[site] = par[i];
-                }
-            }
-            // chains longer than two hops
-            for (uint32_t i = 0; i < kCh; ++i)
-            {
-                if ((slow >> i) & 1u)
-                {
-                    const uint32_t site = row0 + (r0 + i) * bl.stride;
-                    const uint32_t start = SiteOf(bl.base, bl.stride, local[site] & 0x7FFFu);
-                    uint32_t xx = start, p = L[xx];
-                    while (p != xx)
+                    if (p > x)
                     {
-                        if (p > xx)
+                        ok = false;
+                        break;
+                    }
+                    x = p;
+                    p = CachedLoad(&L[x]);
+                }
+                R[q] = x;
+            }
+        }
+        __syncthreads();
+
+        // phase 2: labels and flips, four sites per lane
+        for (uint32_t r0 = 0;;)
+        {
+#pragma unroll
+            for (uint32_t k = 0; k < kCh; ++k)
+            {
+                const uint32_t r = r0 + sub + k * kPass;
+                if (r < bl.nwr)
+                {
+                    const uint32_t site = bl.base + r * bl.stride + col;
+                    const uint32_t lab[4] = {lv[k].x & 0xFFFFu, lv[k].x >> 16, lv[k].y & 0xFFFFu, lv[k].y >> 16};
+                    uint32_t root[4];
+                    uint32_t flip = 0;
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e)
+                    {
+                        uint32_t q = lab[e] & 0x7FFFu;
+                        const bool edge = (lab[e] & kEdgeFlag) != 0u;
+                        if (q >= qmax)
                         {
                             ok = false;
-                            break;
+                            q = 0;
                         }
-                        xx = p;
-                        p = L[xx];
+                        const uint32_t g = R[edge ? q : 0u];
+                        root[e] = edge ? g : SiteOf(bl.base, bl.stride, q);
+                        flip |= (root[e] & 1u) << (8u * e);
                     }
-                    if (p == xx)
-                    {
-                        L[start] = xx;
-                        if (do_flip)
-                            spins[site] = static_cast<uint8_t>(spins[site] ^ (xx & 1u));
-                        if (write_labels)
-                            L[site] = xx;
-                    }
+                    if (do_flip)
+                        *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;
+                    if (write_labels)
+                        *(gptr<u32x4>)(L + site) = u32x4{root[0], root[1], root[2], root[3]};
                 }
             }
+            r0 += kCh * kPass;
+            if (r0 >= bl.nwr)
+                break;
+            load(r0);
         }
         if (!ok)
             (void)__hip_atomic_fetch_or(Glob(d.state) + 3, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -982,10 +1004,11 @@ namespace swmi
         k_merge<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
-    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t flags,
-                    hipStream_t stream)
+    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
+                    const uint32_t flags, hipStream_t stream)
     {
-        k_flip<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks, flags);
+        const size_t lds = static_cast<size_t>(kWavesPerBlock) * max_twr * 64u * sizeof(uint32_t);
+        k_flip<<<XcdGrid(total_blocks), 256, lds, stream>>>(d_desc, d_block_lat, total_blocks, flags);
     }
 
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
