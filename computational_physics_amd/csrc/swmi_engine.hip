@@ -144,6 +144,7 @@ struct swmi_engine
     uint32_t total_blocks = 0;
     uint32_t max_twr = 0;
     bool all_tw64 = true;
+    bool recording = false;   // some lattice records its bonds (selects the inspection build of k_bond_label)
     // single-lattice work lists for the inspection hooks
     DeviceBuffer<swmi::LatDesc> d_one_desc;
     DeviceBuffer<uint32_t> d_zero_blocks;
@@ -230,7 +231,7 @@ struct swmi_engine
     {
         for (uint32_t i = 0; i < n; ++i)
         {
-            swmi::LaunchBondLabel(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, all_tw64, stream);
+            swmi::LaunchBondLabel(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, all_tw64, recording, stream);
             swmi::LaunchMerge(d_desc.ptr, d_block_lat.ptr, total_blocks, stream);
             swmi::LaunchFlip(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, swmi::kFlip | swmi::kAdvance, stream);
         }
@@ -599,6 +600,7 @@ int swmi_record_bonds(swmi_engine* engine, int enable)
             if (!enable)
                 l->bond_dump.reset();
         }
+        engine->recording = enable != 0;
         engine->UploadDescriptors();
     });
 }
@@ -626,7 +628,7 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
             l.bond_in.allocate(l.n);
         Check(hipMemcpyAsync(l.bond_in.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
         engine->OneLattice(l, l.bond_in.ptr);
-        swmi::LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, l.tw == 64, engine->stream);
+        swmi::LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, l.tw == 64, true, engine->stream);
         swmi::LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, engine->stream);
         swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, swmi::kWriteLabels, engine->stream);
         engine->LaunchCheck("label launch");
@@ -689,7 +691,8 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
         for (uint32_t i = 0; i < n_sweeps; ++i)
         {
             Check(hipEventRecord(ev[0], engine->stream), "event");
-            swmi::LaunchBondLabel(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->max_twr, engine->all_tw64, engine->stream);
+            swmi::LaunchBondLabel(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->max_twr, engine->all_tw64,
+                                  engine->recording, engine->stream);
             Check(hipEventRecord(ev[1], engine->stream), "event");
             swmi::LaunchMerge(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->stream);
             Check(hipEventRecord(ev[2], engine->stream), "event");

@@ -1,7 +1,7 @@
 // swmi_internal.h -- device data layout shared by the kernels and the engine.
 //
 // HBM layout of one lattice (n = n0*n1 sites, n % 64 == 0, groups of 64 sites):
-//   spins    uint8[n]        bit 0: spin {0,1}; bits 1-3: the site's current lcg32 parameter selector
+//   spins    uint8[n]        bit 0: spin {0,1}; bits 3-5: the site's current lcg32 parameter selector
 //   rng      uint32[n]       per-site lcg32 state (lane i of group g = site 64g+i)
 //   masks    uint64[3*n/64]  initial (a,c) selector of each lane as three bit planes (read-only)
 //   rot      uint32[n/64]    rotation offset of each group's (a,c) arrays (updated on rotating sweeps)
@@ -32,6 +32,8 @@ namespace swmi
     constexpr uint16_t kEdgeFlag = 0x8000u;  // cluster reaches the block edge
     constexpr uint32_t kRightWords = kWavesPerBlock;
     constexpr uint32_t kBbitWords = 1 + kRightWords;
+    constexpr uint32_t kSelShift = 3;        // spin byte: selector k in bits 3-5 (k << 3 is its 8-byte table offset)
+    constexpr uint32_t kSelMask = 7u << kSelShift;
 
     struct LatDesc
     {
@@ -85,8 +87,9 @@ namespace swmi
 
     // Launch wrappers (swmi_kernels.hip). Work list: block_lat[b] = lattice of global block b.
     // all_tw64: every lattice of the work list has n0 % 64 == 0 (64-wide tiles); selects the specialised kernel
+    // debug: compiles in injected bonds (LatDesc::bond_in) and bond recording (LatDesc::bond_dump)
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
-                         bool all_tw64, hipStream_t stream);
+                         bool all_tw64, bool debug, hipStream_t stream);
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
     void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
                     uint32_t flags, hipStream_t stream);

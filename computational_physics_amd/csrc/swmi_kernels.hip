@@ -6,7 +6,7 @@
 //                 row runs from the masks, unions in an LDS union-find (atomicMin keeps parent <= child,
 //                 so roots are cluster minima), block-local uint16 labels with an edge flag, and the
 //                 block-edge bond bits.
-// Spin bytes: bit 0 is the spin, bits 1-3 hold the site's current lcg32 parameter selector, i.e. the
+// Spin bytes: bit 0 is the spin, bits 3-5 hold the site's current lcg32 parameter selector, i.e. the
 // index k of the Numerical Recipes pair (a_k, c_k) that lane i of its group uses after the group's
 // rotations (sel0[(i + rot) & 63]). The byte is read every sweep anyway, so the generator needs no other
 // per-group data except on the sweeps that rotate (one in 7.5).
@@ -259,7 +259,9 @@ namespace swmi
     // threshold), B (bond to the right neighbour inside the tile), D (bond to the site below). Rows
     // only record their unions (one per overlap interval) in a per-wave LDS queue, which is worked off
     // 64 unions at a time whenever it fills (unions only involve rows already initialised).
-    template <uint32_t TWR_MAX, bool TW64>
+    // DBG: the inspection modes (injected bonds instead of generated ones, bond recording) are compiled in;
+    // the sweep kernel proper has neither branch.
+    template <uint32_t TWR_MAX, bool TW64, bool DBG>
     __global__ __launch_bounds__(256) void k_bond_label(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                         const uint32_t total_blocks)
     {
@@ -272,10 +274,17 @@ namespace swmi
         __shared__ uint32_t Q[kWavesPerBlock * 128];   // per-wave union queues: (a << 16) | b
         __shared__ uint32_t F[kSites / 32];     // roots of clusters that reach the block edge
         __shared__ uint64_t s_first_B[kWavesPerBlock];
+        __shared__ uint64_t AC[8];              // Numerical Recipes pairs (c << 32 | a) by selector
 
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
+        if (threadIdx.x < 8u)
+        {
+            uint32_t a, c;
+            ParamsOf(threadIdx.x, a, c);
+            AC[threadIdx.x] = (static_cast<uint64_t>(c) << 32) | a;
+        }
         const LatDesc& d = descs[block_lat[b]];
         const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
@@ -288,13 +297,13 @@ namespace swmi
         const uint32_t x = bl.sx * tw + lx;
         const uint32_t x_right = (x + 1u == n0) ? 0u : x + 1u;
         const bool rbound = lx == tw - 1u;
-        const bool injected = d.bond_in != nullptr;
+        const bool injected = DBG && d.bond_in != nullptr;
         const uint32_t thr = d.threshold;
         const uint32_t it1 = (injected ? 0u : Glob(d.state)[0]) + 1u;
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
         const bool rot2 = !injected && (it2 % 15u == 0u);
-        const gptr<uint8_t> dump = Glob(d.bond_dump);
+        const gptr<uint8_t> dump = DBG ? Glob(d.bond_dump) : nullptr;
         const gptr<uint8_t> spins = Glob(d.spins);
         const gptr<uint32_t> rng = Glob(d.rng);
         const gptr<uint32_t> rotp = Glob(d.rot);
@@ -330,6 +339,7 @@ namespace swmi
 
         for (uint32_t i = threadIdx.x; i < kSites / 32; i += 256u)
             F[i] = 0;
+        __syncthreads();
 
         uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
         uint32_t qlen = 0;
@@ -337,41 +347,39 @@ namespace swmi
         {
             // spins below the tile (down halo), needed last but fetched first
             const uint32_t y_last = (bl.wyb + r_begin + nrow - 1u) * rpw + lrow;
-            const uint32_t halo_down = (lane < 64u - tw) ? 0u : spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
+            const uint32_t halo_down = spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
 
             uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
             uint32_t prev_site = 0;
             for (uint32_t r0 = 0; r0 < nrow; r0 += kCh)
             {
+                // The loads carry no condition (rows past the tile re-read its last row, values unused), so
+                // that they issue back to back and their latencies overlap.
+                // lane i: row r0 + i of the chunk (for the per-row values fetched one per lane)
+                const uint32_t lrow_ch = r_begin + min(r0 + (lane % kCh), nrow - 1u);
+                // right halo of a 64-wide tile: lane i fetches the column-64 spin of row r0 + i
+                uint32_t h = 0;
+                if (TW64)
+                {
+                    const uint32_t col = (bl.sx * 64u + 64u == n0) ? 0u : bl.sx * 64u + 64u;
+                    h = spins[(bl.wyb + lrow_ch) * n0 + col];
+                }
+                // rotation offsets of the chunk's groups, needed on rotating sweeps only
+                uint32_t g_rot = 0;
+                if (rot1 || rot2)
+                    g_rot = rotp[(bl.base + lrow_ch * bl.stride) >> 6];
                 uint32_t sp[kCh], aux[kCh], rsh[kCh];
 #pragma unroll
                 for (uint32_t i = 0; i < kCh; ++i)
                 {
-                    const uint32_t wr = r_begin + r0 + i;
-                    sp[i] = aux[i] = rsh[i] = 0;
-                    if (r0 + i < nrow)
-                    {
-                        const uint32_t site = bl.base + wr * bl.stride + lane;
-                        sp[i] = spins[site];
-                        aux[i] = injected ? bond_in[site] : rng[site];
-                        if (!TW64 && rbound)
-                            rsh[i] = spins[((bl.wyb + wr) * rpw + lrow) * n0 + x_right];
-                    }
+                    // row base in scalar registers, the lane offset in a vector register
+                    const uint32_t wr = r_begin + min(r0 + i, nrow - 1u);
+                    const uint32_t rowbase = Uniform(bl.base + wr * bl.stride);
+                    sp[i] = (spins + rowbase)[lane];
+                    aux[i] = injected ? (bond_in + rowbase)[lane] : (rng + rowbase)[lane];
+                    rsh[i] = TW64 ? 0u : spins[((bl.wyb + wr) * rpw + lrow) * n0 + x_right];
                 }
-                // rotation offsets of the chunk's groups (lane i: row r0 + i), needed on rotating sweeps only
-                uint32_t g_rot = 0;
-                if ((rot1 || rot2) && lane < kCh && r0 + lane < nrow)
-                    g_rot = rotp[(bl.base + (r_begin + r0 + lane) * bl.stride) >> 6];
-                // right halo of a 64-wide tile: lane i fetches the column-64 spin of row r0 + i
-                uint64_t HR = 0;
-                if (TW64)
-                {
-                    const uint32_t col = (bl.sx * 64u + 64u == n0) ? 0u : bl.sx * 64u + 64u;
-                    uint32_t h = 0;
-                    if (lane < kCh && r0 + lane < nrow)
-                        h = spins[(bl.wyb + r_begin + r0 + lane) * n0 + col];
-                    HR = __ballot((h & 1u) != 0u);
-                }
+                const uint64_t HR = TW64 ? __ballot((h & 1u) != 0u) : 0ull;
 
 #pragma unroll
                 for (uint32_t i = 0; i < kCh; ++i)
@@ -388,27 +396,27 @@ namespace swmi
                             // two LCG32_State<64>::Update of the group of this wave row
                             // (random/src/lcg32.cpp:51-84); a rotation by `shift` moves lane
                             // (i + shift) & 63's selector to lane i
-                            uint32_t k = (sp[i] >> 1) & 7u, st = aux[i], a, c, shift = 0;
+                            uint32_t ks = sp[i] & kSelMask, st = aux[i], shift = 0;
                             if (rot1)
                             {
                                 shift = ShuffleShift(st, it1) & 63u;
-                                k = static_cast<uint32_t>(__shfl(static_cast<int>(k), static_cast<int>((lane + shift) & 63u)));
+                                ks = static_cast<uint32_t>(__shfl(static_cast<int>(ks), static_cast<int>((lane + shift) & 63u)));
                             }
-                            ParamsOf(k, a, c);
-                            st = a * st + c;
+                            uint64_t ac = AC[ks >> kSelShift];
+                            st = static_cast<uint32_t>(ac) * st + static_cast<uint32_t>(ac >> 32);
                             const uint32_t u1 = st;
                             if (rot2)
                             {
                                 const uint32_t s2 = ShuffleShift(st, it2) & 63u;
                                 shift += s2;
-                                k = static_cast<uint32_t>(__shfl(static_cast<int>(k), static_cast<int>((lane + s2) & 63u)));
-                                ParamsOf(k, a, c);
+                                ks = static_cast<uint32_t>(__shfl(static_cast<int>(ks), static_cast<int>((lane + s2) & 63u)));
+                                ac = AC[ks >> kSelShift];
                             }
-                            st = a * st + c;
-                            rng[site] = st;
+                            st = static_cast<uint32_t>(ac) * st + static_cast<uint32_t>(ac >> 32);
+                            (rng + Uniform(site - lane))[lane] = st;
                             if (rot1 || rot2)
                             {
-                                spins[site] = static_cast<uint8_t>((sp[i] & 1u) | (k << 1));
+                                spins[site] = static_cast<uint8_t>((sp[i] & 1u) | ks);
                                 if (lane == 0)
                                     rotp[site >> 6] = (ReadLane(g_rot, i) + shift) & 63u;
                             }
@@ -903,7 +911,7 @@ namespace swmi
         for (uint32_t i = 0; i < 64u; ++i)
         {
             const uint32_t k = static_cast<uint32_t>(((m0 >> i) & 1ull) | (((m1 >> i) & 1ull) << 1) | (((m2 >> i) & 1ull) << 2));
-            sp[i] = static_cast<uint8_t>((sp[i] & 1u) | (k << 1));
+            sp[i] = static_cast<uint8_t>((sp[i] & 1u) | (k << kSelShift));
             if (i > 0)
             {
                 ParamsOf(k, a, c);
@@ -975,28 +983,35 @@ namespace swmi
     // =============================================================================================
     static inline uint32_t XcdGrid(const uint32_t units) { return ((units + kNumXcd - 1) / kNumXcd) * kNumXcd; }
 
-    template <bool TW64>
+    template <bool TW64, bool DBG>
     static void LaunchBondLabelT(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks,
                                  const uint32_t max_twr, hipStream_t stream)
     {
         const uint32_t g = XcdGrid(total_blocks);
         if (max_twr <= 8)
-            k_bond_label<8, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            k_bond_label<8, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
         else if (max_twr <= 16)
-            k_bond_label<16, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            k_bond_label<16, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
         else if (max_twr <= 32)
-            k_bond_label<32, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            k_bond_label<32, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
         else
-            k_bond_label<64, TW64><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            k_bond_label<64, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
-                         const bool all_tw64, hipStream_t stream)
+                         const bool all_tw64, const bool debug, hipStream_t stream)
     {
-        if (all_tw64)
-            LaunchBondLabelT<true>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+        if (debug)
+        {
+            if (all_tw64)
+                LaunchBondLabelT<true, true>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+            else
+                LaunchBondLabelT<false, true>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+        }
+        else if (all_tw64)
+            LaunchBondLabelT<true, false>(d_desc, d_block_lat, total_blocks, max_twr, stream);
         else
-            LaunchBondLabelT<false>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+            LaunchBondLabelT<false, false>(d_desc, d_block_lat, total_blocks, max_twr, stream);
     }
 
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
