@@ -19,8 +19,8 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["SwmiError", "LatticeConfig", "Engine", "library_path", "load_library", "device_info", "copy_calibration",
-           "REFERENCE_TEMPERATURE"]
+__all__ = ["SwmiError", "LatticeConfig", "Engine", "Job", "Server", "library_path", "load_library", "device_info",
+           "copy_calibration", "autotune", "REFERENCE_TEMPERATURE"]
 
 # ising/src/ising_2d.cpp:33, as the float the reference stores it in.
 REFERENCE_TEMPERATURE = float(np.float32(2.2691853142130221))
@@ -48,6 +48,31 @@ class _DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32), ("compute_units", ctypes.c_uint32),
                 ("wavefront_size", ctypes.c_uint32), ("total_memory", ctypes.c_uint64), ("l2_bytes", ctypes.c_uint32),
                 ("clock_khz", ctypes.c_uint32)]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("max_jobs", ctypes.c_uint32), ("chunk_sweeps", ctypes.c_uint32),
+                ("n_sep", ctypes.c_uint32), ("n_lanes", ctypes.c_uint32)]
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("n0", ctypes.c_uint32), ("n1", ctypes.c_uint32), ("temperature", ctypes.c_float),
+                ("seed", ctypes.c_uint32), ("spin_seed", ctypes.c_uint32), ("warmup", ctypes.c_uint32),
+                ("sweeps", ctypes.c_uint32), ("tag", ctypes.c_uint64)]
+
+
+class _JobResult(ctypes.Structure):
+    _fields_ = [("job_id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("n0", ctypes.c_uint32), ("n1", ctypes.c_uint32),
+                ("temperature", ctypes.c_float), ("sweeps_done", ctypes.c_uint32), ("error_flags", ctypes.c_uint32),
+                ("status", ctypes.c_uint32), ("moments", ctypes.c_double * 6), ("energy", ctypes.c_int64),
+                ("magnetization", ctypes.c_int64), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_finish", ctypes.c_double)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("admitted", ctypes.c_uint64), ("finished", ctypes.c_uint64),
+                ("queued", ctypes.c_uint64), ("active", ctypes.c_uint64), ("bytes_in_use", ctypes.c_uint64),
+                ("bytes_pooled", ctypes.c_uint64), ("spin_updates", ctypes.c_uint64), ("clock", ctypes.c_double)]
 
 
 def library_path() -> Path:
@@ -91,6 +116,18 @@ def load_library() -> ctypes.CDLL:
         "swmi_get_error_flags": (ctypes.c_int, [vp, u32, P(u32)]),
         "swmi_time_kernels": (ctypes.c_int, [vp, u32, P(f32)]),
         "swmi_copy_calibration": (ctypes.c_int, [ctypes.c_size_t, u32, P(f32)]),
+        "swmi_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
+        "swmi_get_tile_rows": (ctypes.c_int, [vp, u32, P(u32)]),
+        "swmi_autotune": (ctypes.c_int, [P(_Config), u32, P(u32), P(f32)]),
+        "swmi_set_schedule": (ctypes.c_int, [vp, u32, u32, u32, u32]),
+        "swmi_run_scheduled": (ctypes.c_int, [vp, u32, u32]),
+        "swmi_get_progress": (ctypes.c_int, [vp, u32, P(u32), P(u32)]),
+        "swmi_server_create": (ctypes.c_int, [P(_ServerConfig), P(vp)]),
+        "swmi_server_destroy": (ctypes.c_int, [vp]),
+        "swmi_server_submit": (ctypes.c_int, [vp, P(_Job), P(u64)]),
+        "swmi_server_poll": (ctypes.c_int, [vp, P(_JobResult), u32, P(u32)]),
+        "swmi_server_wait": (ctypes.c_int, [vp, u32, P(u32)]),
+        "swmi_server_get_stats": (ctypes.c_int, [vp, P(_ServerStats)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -120,6 +157,15 @@ def copy_calibration(nbytes: int, repeats: int = 10) -> float:
     ms = ctypes.c_float()
     _check(load_library().swmi_copy_calibration(nbytes, repeats, ctypes.byref(ms)))
     return ms.value
+
+
+def autotune(config: "LatticeConfig", sweeps: int = 20) -> tuple[int, list[float]]:
+    """Time plain sweeps for tile heights 1..64 (wave rows); the fastest is used from now on in this process
+    for lattices of this extent created with tile_rows=0. Returns (best, ms per sweep per height; -1 = skipped)."""
+    best = ctypes.c_uint32()
+    ms = (ctypes.c_float * 7)()
+    _check(load_library().swmi_autotune(ctypes.byref(config._c()), sweeps, ctypes.byref(best), ms))
+    return best.value, list(ms)
 
 
 @dataclass
@@ -203,6 +249,26 @@ class Engine:
     def sweep(self, n: int = 1) -> None:
         _check(self._lib.swmi_sweep(self._h, n))
 
+    def set_graphs(self, enable: bool) -> None:
+        _check(self._lib.swmi_set_graphs(self._h, 1 if enable else 0))
+
+    def tile_rows(self, i: int = 0) -> int:
+        t = ctypes.c_uint32()
+        _check(self._lib.swmi_get_tile_rows(self._h, i, ctypes.byref(t)))
+        return t.value
+
+    def set_schedule(self, i: int, sweeps: int, warmup: int = 0, n_sep: int = 10) -> None:
+        _check(self._lib.swmi_set_schedule(self._h, i, sweeps, warmup, n_sep))
+
+    def run_scheduled(self, n_sweeps: int, n_sep: int = 10) -> None:
+        _check(self._lib.swmi_run_scheduled(self._h, n_sweeps, n_sep))
+
+    def progress(self, i: int = 0) -> tuple[int, int]:
+        """(sweeps done since creation, samples taken)."""
+        a, b = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self._lib.swmi_get_progress(self._h, i, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def sample(self) -> None:
         _check(self._lib.swmi_sample(self._h))
 
@@ -272,3 +338,77 @@ class Engine:
         ms = (ctypes.c_float * 3)()
         _check(self._lib.swmi_time_kernels(self._h, n_sweeps, ms))
         return np.array(ms[:], dtype=np.float64)
+
+
+@dataclass
+class Job:
+    """One served job (fields of swmi_job)."""
+    n0: int
+    n1: int
+    temperature: float
+    seed: int = 0
+    spin_seed: int = 1
+    warmup: int = 0
+    sweeps: int = 1000
+    tag: int = 0
+
+    def _c(self) -> _Job:
+        return _Job(self.n0, self.n1, self.temperature, self.seed & 0xFFFFFFFF, self.spin_seed & 0xFFFFFFFF, self.warmup,
+                    self.sweeps, self.tag)
+
+
+def _result_dict(r: _JobResult) -> dict:
+    return {"job_id": r.job_id, "tag": r.tag, "n0": r.n0, "n1": r.n1, "temperature": r.temperature,
+            "sweeps_done": r.sweeps_done, "error_flags": r.error_flags, "status": r.status,
+            "moments": np.array(r.moments[:], dtype=np.float64), "energy": r.energy, "magnetization": r.magnetization,
+            "t_submit": r.t_submit, "t_admit": r.t_admit, "t_finish": r.t_finish}
+
+
+class Server:
+    """Continuous-batching request loop (swmi_server_*): submit jobs at any time, poll finished results."""
+
+    def __init__(self, memory_budget: int = 0, max_jobs: int = 0, chunk_sweeps: int = 0, n_sep: int = 0, n_lanes: int = 0):
+        self._lib = load_library()
+        cfg = _ServerConfig(memory_budget, max_jobs, chunk_sweeps, n_sep, n_lanes)
+        h = ctypes.c_void_p()
+        _check(self._lib.swmi_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.swmi_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, job: Job) -> int:
+        jid = ctypes.c_uint64()
+        _check(self._lib.swmi_server_submit(self._h, ctypes.byref(job._c()), ctypes.byref(jid)))
+        return jid.value
+
+    def poll(self, max_results: int = 256) -> list[dict]:
+        buf = (_JobResult * max_results)()
+        n = ctypes.c_uint32()
+        _check(self._lib.swmi_server_poll(self._h, buf, max_results, ctypes.byref(n)))
+        return [_result_dict(buf[i]) for i in range(n.value)]
+
+    def wait(self, timeout_ms: int = 60000) -> int:
+        """Block until all submitted jobs are finished (or the timeout); returns the number unfinished."""
+        left = ctypes.c_uint32()
+        _check(self._lib.swmi_server_wait(self._h, timeout_ms, ctypes.byref(left)))
+        return left.value
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(self._lib.swmi_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}

@@ -1,57 +1,44 @@
-// swmi_engine.hip -- host side of the C ABI (include/swmi.h).
+// swmi_engine.hip -- host side of the C ABI (include/swmi.h): engines, work lists, graphs, autotuning.
 //
-// An engine owns a set of lattices (one for the plain path, many for a batch), their device buffers,
-// one HIP stream and the block work list over all of them. Each sweep is three launches over the whole
-// work list, so a batch of lattices of any sizes costs the same number of launches as one lattice.
 // Errors never cross the ABI as exceptions: every entry point returns a status and records a message
 // for swmi_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
-#include <stdexcept>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
-#include "swmi_internal.h"
-#include "../../include/swmi.h"
+#include "swmi_host.h"
 
-namespace
+namespace swmi
 {
-    thread_local std::string g_error;
-
-    struct SwmiError : std::runtime_error
+    namespace
     {
-        int code;
-        SwmiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-    };
+        thread_local std::string g_error;
+
+        std::mutex g_tune_mutex;
+        std::map<std::pair<uint32_t, uint32_t>, uint32_t> g_tuned;   // (n0, n1) -> tile rows
+
+        // Sweeps per captured graph of plain sweeps; a sweep loop of fewer sweeps runs eagerly.
+        constexpr uint32_t kGraphSweeps = 16;
+        // Captured measurement graphs hold groups of (n_sep sweeps + sample) up to about this many sweeps.
+        constexpr uint32_t kGraphGroupSweeps = 64;
+        constexpr size_t kMaxGraphs = 4;
+    }
+
+    void SetLastError(const std::string& m) { g_error = m; }
 
     void Check(hipError_t e, const char* what)
     {
         if (e != hipSuccess)
             throw SwmiError(SWMI_ERR_HIP, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
-    }
-
-    template <typename F>
-    int Guard(F&& f)
-    {
-        try
-        {
-            f();
-            return SWMI_OK;
-        }
-        catch (const SwmiError& e)
-        {
-            g_error = e.what();
-            return e.code;
-        }
-        catch (const std::exception& e)
-        {
-            g_error = e.what();
-            return SWMI_ERR_INVALID;
-        }
     }
 
     // p_add exactly as the reference computes it (ising/src/swendsen_wang.cpp:100), all in float with
@@ -78,192 +65,6 @@ namespace
         return lo;
     }
 
-    template <typename T>
-    struct DeviceBuffer
-    {
-        T* ptr = nullptr;
-        size_t count = 0;
-        DeviceBuffer() = default;
-        DeviceBuffer(const DeviceBuffer&) = delete;
-        DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-        ~DeviceBuffer() { reset(); }
-        void allocate(size_t n)
-        {
-            reset();
-            if (n == 0)
-                return;
-            if (hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)) != hipSuccess)
-            {
-                ptr = nullptr;
-                (void)hipGetLastError();
-                throw SwmiError(SWMI_ERR_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
-            }
-            count = n;
-        }
-        void reset()
-        {
-            if (ptr)
-                (void)hipFree(ptr);
-            ptr = nullptr;
-            count = 0;
-        }
-    };
-
-    struct Lattice
-    {
-        swmi_lattice_config cfg{};
-        uint64_t n = 0;
-        uint32_t n_groups = 0;
-        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0;
-        float p_add = 0.0f;
-        uint64_t threshold = 0;
-        DeviceBuffer<uint8_t> spins;
-        DeviceBuffer<uint32_t> rng;
-        DeviceBuffer<uint64_t> masks;
-        DeviceBuffer<uint32_t> rot;
-        DeviceBuffer<uint16_t> local;
-        DeviceBuffer<uint32_t> parent;
-        DeviceBuffer<uint64_t> bbits;
-        DeviceBuffer<uint32_t> state;
-        DeviceBuffer<int32_t> partials;
-        DeviceBuffer<uint16_t> edge_list;
-        DeviceBuffer<uint32_t> edge_count;
-        DeviceBuffer<uint8_t> bond_dump;
-        DeviceBuffer<uint8_t> bond_in;
-    };
-}
-
-struct swmi_engine
-{
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::vector<std::unique_ptr<Lattice>> lattices;
-    std::vector<swmi::LatDesc> descs;
-    DeviceBuffer<swmi::LatDesc> d_desc;
-    DeviceBuffer<uint32_t> d_block_lat;
-    uint32_t total_blocks = 0;
-    uint32_t max_twr = 0;
-    bool all_tw64 = true;
-    bool recording = false;   // some lattice records its bonds (selects the inspection build of k_bond_label)
-    // single-lattice work lists for the inspection hooks
-    DeviceBuffer<swmi::LatDesc> d_one_desc;
-    DeviceBuffer<uint32_t> d_zero_blocks;
-
-    ~swmi_engine()
-    {
-        if (stream)
-        {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
-
-    Lattice& At(uint32_t i)
-    {
-        if (i >= lattices.size())
-            throw SwmiError(SWMI_ERR_INVALID, "lattice index " + std::to_string(i) + " out of range");
-        return *lattices[i];
-    }
-
-    swmi::LatDesc Desc(const Lattice& l, uint32_t block_base) const
-    {
-        swmi::LatDesc d{};
-        d.spins = l.spins.ptr;
-        d.rng = l.rng.ptr;
-        d.masks = l.masks.ptr;
-        d.rot = l.rot.ptr;
-        d.local = l.local.ptr;
-        d.parent = l.parent.ptr;
-        d.bbits = l.bbits.ptr;
-        d.state = l.state.ptr;
-        d.bond_dump = l.bond_dump.ptr;
-        d.bond_in = nullptr;
-        d.partials = l.partials.ptr;
-        d.edge_list = l.edge_list.ptr;
-        d.edge_count = l.edge_count.ptr;
-        d.edge_cap = l.edge_cap;
-        d.n0 = l.cfg.n0;
-        d.n1 = l.cfg.n1;
-        d.tw = l.tw;
-        d.rpw = l.rpw;
-        d.twr = l.twr;
-        d.tiles_x = l.tiles_x;
-        d.blocks_y = l.blocks_y;
-        d.nwr_y = l.nwr_y;
-        d.threshold = static_cast<uint32_t>(l.threshold);
-        d.block_base = block_base;
-        d.n_blocks = l.n_blocks;
-        d.flip_rule = l.cfg.flip_rule;
-        return d;
-    }
-
-    void UploadDescriptors()
-    {
-        descs.clear();
-        uint32_t base = 0;
-        max_twr = 1;
-        all_tw64 = true;
-        std::vector<uint32_t> block_lat;
-        for (uint32_t i = 0; i < lattices.size(); ++i)
-        {
-            const Lattice& l = *lattices[i];
-            descs.push_back(Desc(l, base));
-            block_lat.insert(block_lat.end(), l.n_blocks, i);
-            base += l.n_blocks;
-            max_twr = std::max(max_twr, l.twr);
-            all_tw64 = all_tw64 && l.tw == 64;
-        }
-        total_blocks = base;
-        if (d_desc.count != descs.size())
-            d_desc.allocate(descs.size());
-        if (d_block_lat.count != block_lat.size())
-            d_block_lat.allocate(block_lat.size());
-        Check(hipMemcpyAsync(d_desc.ptr, descs.data(), descs.size() * sizeof(swmi::LatDesc), hipMemcpyHostToDevice, stream),
-              "descriptor upload");
-        Check(hipMemcpyAsync(d_block_lat.ptr, block_lat.data(), block_lat.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream),
-              "work list upload");
-        Check(hipStreamSynchronize(stream), "descriptor upload");
-    }
-
-    void LaunchCheck(const char* what) { Check(hipGetLastError(), what); }
-
-    void Sweeps(uint32_t n)
-    {
-        for (uint32_t i = 0; i < n; ++i)
-        {
-            swmi::LaunchBondLabel(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, all_tw64, recording, stream);
-            swmi::LaunchMerge(d_desc.ptr, d_block_lat.ptr, total_blocks, stream);
-            swmi::LaunchFlip(d_desc.ptr, d_block_lat.ptr, total_blocks, max_twr, swmi::kFlip | swmi::kAdvance, stream);
-        }
-        LaunchCheck("sweep launch");
-    }
-
-    void Sample()
-    {
-        swmi::LaunchObservables(d_desc.ptr, d_block_lat.ptr, total_blocks, stream);
-        swmi::LaunchFinalizeSamples(d_desc.ptr, static_cast<uint32_t>(lattices.size()), stream);
-        LaunchCheck("sample launch");
-    }
-
-    // Work list holding only lattice i (block_base 0), for the inspection hooks.
-    void OneLattice(const Lattice& l, const uint8_t* bond_in)
-    {
-        swmi::LatDesc d = Desc(l, 0);
-        d.bond_in = bond_in;
-        if (!d_one_desc.ptr)
-            d_one_desc.allocate(1);
-        if (d_zero_blocks.count < l.n_blocks)
-        {
-            d_zero_blocks.allocate(l.n_blocks);
-            Check(hipMemsetAsync(d_zero_blocks.ptr, 0, l.n_blocks * sizeof(uint32_t), stream), "work list");
-        }
-        Check(hipMemcpyAsync(d_one_desc.ptr, &d, sizeof(d), hipMemcpyHostToDevice, stream), "descriptor upload");
-        Check(hipStreamSynchronize(stream), "descriptor upload");
-    }
-};
-
-namespace
-{
     void ValidateConfig(const swmi_lattice_config& c)
     {
         const auto bad = [](const std::string& m) { throw SwmiError(SWMI_ERR_INVALID, m); };
@@ -280,50 +81,8 @@ namespace
             bad("temperature must be positive and finite");
         if (c.flip_rule != SWMI_FLIP_PARITY)
             bad("unknown flip rule " + std::to_string(c.flip_rule));
-    }
-
-    void SetupLattice(Lattice& l, const swmi_lattice_config& c, hipStream_t stream)
-    {
-        l.cfg = c;
-        l.n = static_cast<uint64_t>(c.n0) * c.n1;
-        l.n_groups = static_cast<uint32_t>(l.n / 64);
-        l.tw = std::min<uint32_t>(c.n0, 64);
-        l.rpw = 64 / l.tw;
-        l.tiles_x = c.n0 / l.tw;
-        l.nwr_y = c.n1 / l.rpw;
-        uint32_t twr = c.tile_rows ? c.tile_rows : 16u;
-        twr = std::min<uint32_t>(twr, 64u);
-        twr = std::min<uint32_t>(twr, l.tw);      // lattice rows per tile <= 64 (right-edge bit mask)
-        twr = std::min<uint32_t>(twr, l.nwr_y);
-        l.twr = std::max<uint32_t>(twr, 1u);
-        l.blocks_y = (l.nwr_y + swmi::kWavesPerBlock * l.twr - 1) / (swmi::kWavesPerBlock * l.twr);
-        l.n_blocks = l.tiles_x * l.blocks_y;
-        // distinct clusters reaching a block's boundary <= boundary sites (2 columns + 2 rows)
-        l.edge_cap = std::min<uint32_t>(l.twr * 64u, 2u * l.tw + 2u * swmi::kWavesPerBlock * l.twr * l.rpw);
-        l.p_add = BondProbability(c.temperature);
-        l.threshold = BondThreshold(l.p_add);
-
-        l.spins.allocate(l.n);
-        l.rng.allocate(l.n);
-        l.masks.allocate(3ull * l.n_groups);
-        l.rot.allocate(l.n_groups);
-        l.local.allocate(l.n);
-        l.parent.allocate(l.n);
-        l.bbits.allocate(static_cast<size_t>(swmi::kBbitWords) * l.n_blocks);
-        const size_t state_words = swmi::kLatStateWords + 4ull * c.max_samples;
-        l.state.allocate(state_words);
-        l.partials.allocate(2ull * swmi::kWavesPerBlock * l.n_blocks);
-        l.edge_list.allocate(static_cast<size_t>(swmi::kWavesPerBlock) * l.n_blocks * l.edge_cap);
-        l.edge_count.allocate(static_cast<size_t>(swmi::kWavesPerBlock) * l.n_blocks);
-        Check(hipMemsetAsync(l.edge_count.ptr, 0, l.edge_count.count * 4, stream), "edge list init");
-        Check(hipMemsetAsync(l.state.ptr, 0, state_words * 4, stream), "state init");
-        Check(hipMemcpyAsync(reinterpret_cast<uint8_t*>(l.state.ptr) + 8, &c.max_samples, 4, hipMemcpyHostToDevice, stream),
-              "state init");
-        swmi::LaunchHotStart(l.spins.ptr, l.n, c.spin_seed, stream);
-        swmi::LaunchInitRng(l.spins.ptr, l.rng.ptr, l.masks.ptr, l.rot.ptr, l.n_groups, c.seed, stream);
-        swmi::LaunchIota(l.parent.ptr, l.n, stream);
-        Check(hipMemsetAsync(l.local.ptr, 0, l.n * sizeof(uint16_t), stream), "label init");
-        Check(hipGetLastError(), "initialisation launch");
+        if (c.max_samples > (1u << 28))
+            bad("max_samples too large");
     }
 
     void RequireGfx950(int device)
@@ -339,13 +98,380 @@ namespace
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             throw SwmiError(SWMI_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", the engine is built for gfx950");
     }
+
+    uint32_t DefaultTileRows(const uint32_t n0, const uint32_t n1)
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        const auto it = g_tuned.find({n0, n1});
+        return it == g_tuned.end() ? 16u : it->second;
+    }
+
+    uint32_t TwrClass(const uint32_t twr)
+    {
+        return twr <= 8 ? 8u : (twr <= 16 ? 16u : (twr <= 32 ? 32u : 64u));
+    }
+
+    Geometry GeometryOf(const uint32_t n0, const uint32_t n1, const uint32_t tile_rows)
+    {
+        Geometry g;
+        g.tw = std::min<uint32_t>(n0, 64);
+        g.rpw = 64 / g.tw;
+        g.tiles_x = n0 / g.tw;
+        g.nwr_y = n1 / g.rpw;
+        uint32_t twr = tile_rows ? tile_rows : DefaultTileRows(n0, n1);
+        twr = std::min<uint32_t>(twr, 64u);
+        twr = std::min<uint32_t>(twr, g.tw);      // lattice rows per tile <= 64 (right-edge bit mask)
+        twr = std::min<uint32_t>(twr, g.nwr_y);
+        g.twr = std::max<uint32_t>(twr, 1u);
+        g.blocks_y = (g.nwr_y + kWavesPerBlock * g.twr - 1) / (kWavesPerBlock * g.twr);
+        g.n_blocks = g.tiles_x * g.blocks_y;
+        // distinct clusters reaching a block's boundary <= boundary sites (2 columns + 2 rows)
+        g.edge_cap = std::min<uint32_t>(g.twr * 64u, 2u * g.tw + 2u * kWavesPerBlock * g.twr * g.rpw);
+        return g;
+    }
+
+    uint64_t Lattice::ResidentBytes(const swmi_lattice_config& c)
+    {
+        const Geometry g = GeometryOf(c.n0, c.n1, c.tile_rows);
+        const uint64_t n = static_cast<uint64_t>(c.n0) * c.n1, groups = n / 64;
+        return n * (1 + 4 + 2 + 4) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
+               4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
+               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4);
+    }
+
+    bool Lattice::SameShape(const swmi_lattice_config& c) const
+    {
+        const Geometry h = GeometryOf(c.n0, c.n1, c.tile_rows);
+        return cfg.n0 == c.n0 && cfg.n1 == c.n1 && g.twr == h.twr && cfg.max_samples == c.max_samples;
+    }
+
+    uint64_t Lattice::Bytes() const
+    {
+        return spins.bytes() + rng.bytes() + masks.bytes() + rot.bytes() + local.bytes() + parent.bytes() + bbits.bytes() +
+               state.bytes() + partials.bytes() + edge_list.bytes() + edge_count.bytes() + bond_dump.bytes() + bond_in.bytes();
+    }
+
+    void Lattice::Allocate(const swmi_lattice_config& c)
+    {
+        cfg = c;
+        n = static_cast<uint64_t>(c.n0) * c.n1;
+        n_groups = static_cast<uint32_t>(n / 64);
+        g = GeometryOf(c.n0, c.n1, c.tile_rows);
+        spins.allocate(n);
+        rng.allocate(n);
+        masks.allocate(3ull * n_groups);
+        rot.allocate(n_groups);
+        local.allocate(n);
+        parent.allocate(n);
+        bbits.allocate(static_cast<size_t>(kBbitWords) * g.n_blocks);
+        state_words = kLatStateWords + 4u * c.max_samples;
+        state.allocate(state_words);
+        partials.allocate(2ull * kWavesPerBlock * g.n_blocks);
+        edge_list.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
+        edge_count.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
+    }
+
+    void Lattice::Initialize(const swmi_lattice_config& c, hipStream_t stream)
+    {
+        cfg.temperature = c.temperature;
+        cfg.seed = c.seed;
+        cfg.spin_seed = c.spin_seed;
+        cfg.flip_rule = c.flip_rule;
+        p_add = BondProbability(c.temperature);
+        threshold = BondThreshold(p_add);
+        bond_dump.reset();
+        Check(hipMemsetAsync(edge_count.ptr, 0, edge_count.bytes(), stream), "edge list init");
+        LaunchInitState(state.ptr, state_words, c.max_samples, 0xFFFFFFFFu, 0u, 0u, stream);
+        LaunchHotStart(spins.ptr, n, c.spin_seed, stream);
+        LaunchInitRng(spins.ptr, rng.ptr, masks.ptr, rot.ptr, n_groups, c.seed, stream);
+        LaunchIota(parent.ptr, n, stream);
+        Check(hipMemsetAsync(local.ptr, 0, local.bytes(), stream), "label init");
+        Check(hipGetLastError(), "initialisation launch");
+    }
+}
+
+using namespace swmi;
+
+swmi_engine::swmi_engine()
+{
+    RequireGfx950(0);
+    Check(hipSetDevice(0), "hipSetDevice");
+    Check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+swmi_engine::~swmi_engine()
+{
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    DropGraphs();
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+Lattice& swmi_engine::At(uint32_t i)
+{
+    if (i >= lattices.size() || !lattices[i])
+        throw SwmiError(SWMI_ERR_INVALID, "lattice index " + std::to_string(i) + " out of range");
+    return *lattices[i];
+}
+
+uint64_t swmi_engine::TotalSites() const
+{
+    uint64_t s = 0;
+    for (const auto& l : lattices)
+        if (l)
+            s += l->n;
+    return s;
+}
+
+LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
+{
+    LatDesc d{};
+    d.spins = l.spins.ptr;
+    d.rng = l.rng.ptr;
+    d.masks = l.masks.ptr;
+    d.rot = l.rot.ptr;
+    d.local = l.local.ptr;
+    d.parent = l.parent.ptr;
+    d.bbits = l.bbits.ptr;
+    d.state = l.state.ptr;
+    d.bond_dump = l.bond_dump.ptr;
+    d.bond_in = nullptr;
+    d.partials = l.partials.ptr;
+    d.edge_list = l.edge_list.ptr;
+    d.edge_count = l.edge_count.ptr;
+    d.edge_cap = l.g.edge_cap;
+    d.n0 = l.cfg.n0;
+    d.n1 = l.cfg.n1;
+    d.tw = l.g.tw;
+    d.rpw = l.g.rpw;
+    d.twr = l.g.twr;
+    d.tiles_x = l.g.tiles_x;
+    d.blocks_y = l.g.blocks_y;
+    d.nwr_y = l.g.nwr_y;
+    d.threshold = static_cast<uint32_t>(l.threshold);
+    d.block_base = block_base;
+    d.n_blocks = l.g.n_blocks;
+    d.flip_rule = l.cfg.flip_rule;
+    return d;
+}
+
+void swmi_engine::DropGraphs()
+{
+    for (auto& g : graphs)
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
+void swmi_engine::Rebuild()
+{
+    DropGraphs();
+    // slots are compacted: the descriptor array has no holes
+    lattices.erase(std::remove(lattices.begin(), lattices.end(), nullptr), lattices.end());
+    // class of a lattice: (64-wide tiles, k_bond_label tile-height bound)
+    std::vector<KernelClass> next;
+    std::vector<LatDesc> d(lattices.size());
+    for (uint32_t i = 0; i < lattices.size(); ++i)
+    {
+        const Lattice& l = *lattices[i];
+        const bool tw64 = l.g.tw == 64;
+        const uint32_t t = TwrClass(l.g.twr);
+        auto it = std::find_if(next.begin(), next.end(), [&](const KernelClass& k) { return k.tw64 == tw64 && k.twr_t == t; });
+        if (it == next.end())
+        {
+            next.emplace_back();
+            it = next.end() - 1;
+            it->tw64 = tw64;
+            it->twr_t = t;
+        }
+        d[i] = Desc(l, static_cast<uint32_t>(it->block_lat.size()));
+        it->block_lat.insert(it->block_lat.end(), l.g.n_blocks, i);
+        it->max_twr = std::max(it->max_twr, l.g.twr);
+    }
+    // reuse the device work-list buffers of the previous classes where the sizes allow
+    for (auto& k : next)
+    {
+        k.total_blocks = static_cast<uint32_t>(k.block_lat.size());
+        auto old = std::find_if(classes.begin(), classes.end(), [&](KernelClass& o) {
+            return o.d_block_lat.ptr && o.d_block_lat.count >= k.block_lat.size();
+        });
+        if (old != classes.end())
+        {
+            std::swap(k.d_block_lat.ptr, old->d_block_lat.ptr);
+            std::swap(k.d_block_lat.count, old->d_block_lat.count);
+        }
+        else
+            k.d_block_lat.allocate(std::max<size_t>(k.block_lat.size(), 1));
+    }
+    descs.swap(d);
+    if (d_desc.count < std::max<size_t>(descs.size(), 1))
+        d_desc.allocate(std::max<size_t>(descs.size(), 1));
+    classes.swap(next);   // the old buffers die with `next` (after the stream has used them: hipFree synchronises)
+    if (!descs.empty())
+        Check(hipMemcpyAsync(d_desc.ptr, descs.data(), descs.size() * sizeof(LatDesc), hipMemcpyHostToDevice, stream),
+              "descriptor upload");
+    for (auto& k : classes)
+        Check(hipMemcpyAsync(k.d_block_lat.ptr, k.block_lat.data(), k.block_lat.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             stream),
+              "work list upload");
+}
+
+void swmi_engine::LaunchCheck(const char* what) { Check(hipGetLastError(), what); }
+
+void swmi_engine::EnqueueSweeps(uint32_t n)
+{
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        for (auto& k : classes)
+            LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, recording, stream);
+        for (auto& k : classes)
+            LaunchMerge(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, stream);
+        for (auto& k : classes)
+            LaunchFlip(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, kFlip | kAdvance, stream);
+    }
+    LaunchCheck("sweep launch");
+}
+
+void swmi_engine::EnqueueSample(uint32_t mode)
+{
+    if (descs.empty())
+        return;
+    for (auto& k : classes)
+        LaunchObservables(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, mode, stream);
+    LaunchFinalizeSamples(d_desc.ptr, static_cast<uint32_t>(descs.size()), mode, stream);
+    LaunchCheck("sample launch");
+}
+
+void swmi_engine::Run(uint32_t n, uint32_t sep, int32_t sample_mode)
+{
+    if (classes.empty() || n == 0)
+        return;
+    const bool sampled = sample_mode >= 0;
+    if (sampled && sep > kGraphGroupSweeps)
+    {
+        // long groups: graphs of plain sweeps, eager samples
+        for (uint32_t i = 0; i < n; i += sep)
+        {
+            Run(sep, 0, -1);
+            EnqueueSample(static_cast<uint32_t>(sample_mode));
+        }
+        return;
+    }
+    // one replay unit: `unit` sweeps, or `groups` groups of (sep sweeps + sample)
+    const uint32_t groups = sampled ? std::max<uint32_t>(1u, kGraphGroupSweeps / sep) : 1u;
+    const uint32_t unit = sampled ? groups * sep : kGraphSweeps;
+    const uint32_t total_groups = sampled ? (n + sep - 1) / sep : 0u;   // ising_2d.cpp:132: i += N_Sep while i < n
+    const uint32_t replays = sampled ? total_groups / groups : n / unit;
+    const auto enqueue_unit = [&](uint32_t g_count) {
+        if (sampled)
+            for (uint32_t g = 0; g < g_count; ++g)
+            {
+                EnqueueSweeps(sep);
+                EnqueueSample(static_cast<uint32_t>(sample_mode));
+            }
+        else
+            EnqueueSweeps(g_count);
+    };
+    if (use_graphs && replays >= 2)
+    {
+        GraphEntry* entry = nullptr;
+        for (auto& g : graphs)
+            if (g.sweeps == unit && g.sample == (sampled ? sample_mode : -1) && g.sep == (sampled ? sep : 0u))
+                entry = &g;
+        if (!entry)
+        {
+            if (graphs.size() >= kMaxGraphs)
+                DropGraphs();
+            hipGraph_t graph = nullptr;
+            Check(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture");
+            try
+            {
+                enqueue_unit(sampled ? groups : unit);
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(stream, &graph);
+                if (graph)
+                    (void)hipGraphDestroy(graph);
+                throw;
+            }
+            Check(hipStreamEndCapture(stream, &graph), "graph capture");
+            GraphEntry ge;
+            ge.sweeps = unit;
+            ge.sample = sampled ? sample_mode : -1;
+            ge.sep = sampled ? sep : 0u;
+            const hipError_t err = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            Check(err, "graph instantiate");
+            graphs.push_back(ge);
+            entry = &graphs.back();
+        }
+        for (uint32_t r = 0; r < replays; ++r)
+            Check(hipGraphLaunch(entry->exec, stream), "graph launch");
+        if (sampled)
+            enqueue_unit(total_groups - replays * groups);
+        else
+            enqueue_unit(n - replays * unit);
+        return;
+    }
+    enqueue_unit(sampled ? total_groups : n);
+}
+
+// Work list holding only lattice l (block_base 0), for the inspection hooks.
+void swmi_engine::OneLattice(const Lattice& l, const uint8_t* bond_in)
+{
+    LatDesc d = Desc(l, 0);
+    d.bond_in = bond_in;
+    if (!d_one_desc.ptr)
+        d_one_desc.allocate(1);
+    if (d_zero_blocks.count < l.g.n_blocks)
+    {
+        d_zero_blocks.allocate(l.g.n_blocks);
+        Check(hipMemsetAsync(d_zero_blocks.ptr, 0, l.g.n_blocks * sizeof(uint32_t), stream), "work list");
+    }
+    Check(hipMemcpyAsync(d_one_desc.ptr, &d, sizeof(d), hipMemcpyHostToDevice, stream), "descriptor upload");
+    Check(hipStreamSynchronize(stream), "descriptor upload");
+}
+
+namespace
+{
+    swmi_engine* Eng(swmi_engine* e)
+    {
+        if (!e)
+            throw SwmiError(SWMI_ERR_INVALID, "null engine");
+        return e;
+    }
+
+    // Average device time of a plain sweep of one lattice of this configuration (HIP events).
+    float TimeSweeps(const swmi_lattice_config& c, const uint32_t sweeps)
+    {
+        swmi_engine e;
+        e.lattices.push_back(std::make_unique<Lattice>());
+        e.lattices.back()->Allocate(c);
+        e.lattices.back()->Initialize(c, e.stream);
+        e.Rebuild();
+        e.Run(4, 0, -1);
+        hipEvent_t a, b;
+        Check(hipEventCreate(&a), "event");
+        Check(hipEventCreate(&b), "event");
+        Check(hipEventRecord(a, e.stream), "event");
+        e.Run(sweeps, 0, -1);
+        Check(hipEventRecord(b, e.stream), "event");
+        Check(hipEventSynchronize(b), "event");
+        float ms = 0;
+        Check(hipEventElapsedTime(&ms, a, b), "event");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return ms / sweeps;
+    }
 }
 
 extern "C" {
 
 const char* swmi_last_error(void) { return g_error.c_str(); }
 
-const char* swmi_version(void) { return "swmi 0.1 (gfx950)"; }
+const char* swmi_version(void) { return "swmi 0.2 (gfx950)"; }
 
 int swmi_get_device_info(swmi_device_info* out)
 {
@@ -374,16 +500,14 @@ int swmi_create(const swmi_lattice_config* configs, uint32_t n_lattices, swmi_en
         *out = nullptr;
         for (uint32_t i = 0; i < n_lattices; ++i)
             ValidateConfig(configs[i]);
-        RequireGfx950(0);
-        Check(hipSetDevice(0), "hipSetDevice");
         auto e = std::make_unique<swmi_engine>();
-        Check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (uint32_t i = 0; i < n_lattices; ++i)
         {
             e->lattices.push_back(std::make_unique<Lattice>());
-            SetupLattice(*e->lattices.back(), configs[i], e->stream);
+            e->lattices.back()->Allocate(configs[i]);
+            e->lattices.back()->Initialize(configs[i], e->stream);
         }
-        e->UploadDescriptors();
+        e->Rebuild();
         Check(hipStreamSynchronize(e->stream), "initialisation");
         *out = e.release();
     });
@@ -405,26 +529,22 @@ int swmi_num_lattices(const swmi_engine* engine, uint32_t* out)
 
 int swmi_synchronize(swmi_engine* engine)
 {
-    return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        Check(hipStreamSynchronize(engine->stream), "synchronize");
-    });
+    return Guard([&] { Check(hipStreamSynchronize(Eng(engine)->stream), "synchronize"); });
 }
 
 int swmi_upload_spins(swmi_engine* engine, uint32_t lattice, const uint8_t* spins)
 {
     return Guard([&] {
-        if (!engine || !spins)
+        if (!spins)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         for (uint64_t i = 0; i < l.n; ++i)
             if (spins[i] > 1)
                 throw SwmiError(SWMI_ERR_INVALID, "spins must be 0 or 1");
         DeviceBuffer<uint8_t> tmp;
         tmp.allocate(l.n);
         Check(hipMemcpyAsync(tmp.ptr, spins, l.n, hipMemcpyHostToDevice, engine->stream), "spin upload");
-        swmi::LaunchSetSpins(l.spins.ptr, tmp.ptr, l.n, engine->stream);
+        LaunchSetSpins(l.spins.ptr, tmp.ptr, l.n, engine->stream);
         engine->LaunchCheck("spin upload");
         Check(hipStreamSynchronize(engine->stream), "spin upload");
     });
@@ -433,12 +553,12 @@ int swmi_upload_spins(swmi_engine* engine, uint32_t lattice, const uint8_t* spin
 int swmi_download_spins(swmi_engine* engine, uint32_t lattice, uint8_t* spins)
 {
     return Guard([&] {
-        if (!engine || !spins)
+        if (!spins)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         Check(hipMemcpyAsync(spins, l.spins.ptr, l.n, hipMemcpyDeviceToHost, engine->stream), "spin download");
         Check(hipStreamSynchronize(engine->stream), "spin download");
-        for (uint64_t i = 0; i < l.n; ++i)   // bits 1-3 carry the lcg32 selectors
+        for (uint64_t i = 0; i < l.n; ++i)   // bits 3-5 carry the lcg32 selectors
             spins[i] &= 1u;
     });
 }
@@ -446,16 +566,15 @@ int swmi_download_spins(swmi_engine* engine, uint32_t lattice, uint8_t* spins)
 int swmi_set_temperature(swmi_engine* engine, uint32_t lattice, float temperature)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         if (!(temperature > 0.0f) || !std::isfinite(temperature))
             throw SwmiError(SWMI_ERR_INVALID, "temperature must be positive and finite");
+        Check(hipStreamSynchronize(engine->stream), "set temperature");
         l.cfg.temperature = temperature;
         l.p_add = BondProbability(temperature);
         l.threshold = BondThreshold(l.p_add);
+        engine->Rebuild();
         Check(hipStreamSynchronize(engine->stream), "set temperature");
-        engine->UploadDescriptors();
     });
 }
 
@@ -474,47 +593,74 @@ int swmi_get_threshold(const swmi_engine* engine, uint32_t lattice, float* p_add
     });
 }
 
-int swmi_sweep(swmi_engine* engine, uint32_t n_sweeps)
+int swmi_get_tile_rows(const swmi_engine* engine, uint32_t lattice, uint32_t* tile_rows)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        engine->Sweeps(n_sweeps);
+        if (!engine || !tile_rows)
+            throw SwmiError(SWMI_ERR_INVALID, "null argument");
+        if (lattice >= engine->lattices.size())
+            throw SwmiError(SWMI_ERR_INVALID, "lattice index out of range");
+        *tile_rows = engine->lattices[lattice]->g.twr;
     });
+}
+
+int swmi_sweep(swmi_engine* engine, uint32_t n_sweeps)
+{
+    return Guard([&] { Eng(engine)->Run(n_sweeps, 0, -1); });
 }
 
 int swmi_sample(swmi_engine* engine)
 {
-    return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        engine->Sample();
-    });
+    return Guard([&] { Eng(engine)->EnqueueSample(kSampleAlways); });
 }
 
 int swmi_run(swmi_engine* engine, uint32_t n_sweeps, uint32_t n_sep)
 {
     return Guard([&] {
-        if (!engine || n_sep == 0)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine or n_sep == 0");
+        if (n_sep == 0)
+            throw SwmiError(SWMI_ERR_INVALID, "n_sep == 0");
         // ising/src/ising_2d.cpp:132-146: n_sep sweeps, then a sample, while i < n_sweeps
-        for (uint32_t i = 0; i < n_sweeps; i += n_sep)
-        {
-            engine->Sweeps(n_sep);
-            engine->Sample();
-        }
+        Eng(engine)->Run(n_sweeps, n_sep, kSampleAlways);
+    });
+}
+
+int swmi_set_schedule(swmi_engine* engine, uint32_t lattice, uint32_t sweeps, uint32_t warmup, uint32_t n_sep)
+{
+    return Guard([&] {
+        Lattice& l = Eng(engine)->At(lattice);
+        if (n_sep == 0 || (sweeps && warmup > sweeps) || sweeps > (1u << 30))
+            throw SwmiError(SWMI_ERR_INVALID, "schedule needs n_sep > 0, warmup <= sweeps and sweeps <= 2^30");
+        LaunchSetSchedule(l.state.ptr, sweeps, warmup, n_sep, engine->stream);
+        engine->LaunchCheck("schedule launch");
+    });
+}
+
+int swmi_run_scheduled(swmi_engine* engine, uint32_t n_sweeps, uint32_t n_sep)
+{
+    return Guard([&] {
+        if (n_sep == 0)
+            throw SwmiError(SWMI_ERR_INVALID, "n_sep == 0");
+        Eng(engine)->Run(n_sweeps, n_sep, kSampleDue);
+    });
+}
+
+int swmi_set_graphs(swmi_engine* engine, int enable)
+{
+    return Guard([&] {
+        Check(hipStreamSynchronize(Eng(engine)->stream), "set graphs");
+        engine->use_graphs = enable != 0;
+        engine->DropGraphs();
     });
 }
 
 int swmi_observables(swmi_engine* engine, uint32_t lattice, int64_t* energy, int64_t* magnetization)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        Lattice& l = engine->At(lattice);
-        swmi::LaunchObservables(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->stream);
+        Lattice& l = Eng(engine)->At(lattice);
+        engine->OneLattice(l, nullptr);
+        LaunchObservables(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, kSampleAlways, engine->stream);
         engine->LaunchCheck("observables launch");
-        std::vector<int32_t> p(2ull * swmi::kWavesPerBlock * l.n_blocks);
+        std::vector<int32_t> p(2ull * kWavesPerBlock * l.g.n_blocks);
         Check(hipMemcpyAsync(p.data(), l.partials.ptr, p.size() * 4, hipMemcpyDeviceToHost, engine->stream), "observables");
         Check(hipStreamSynchronize(engine->stream), "observables");
         int64_t e = 0, m = 0;
@@ -533,23 +679,35 @@ int swmi_observables(swmi_engine* engine, uint32_t lattice, int64_t* energy, int
 int swmi_get_moments(swmi_engine* engine, uint32_t lattice, double* moments)
 {
     return Guard([&] {
-        if (!engine || !moments)
+        if (!moments)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
-        swmi::LatState st{};
+        Lattice& l = Eng(engine)->At(lattice);
+        LatState st{};
         Check(hipMemcpyAsync(&st, l.state.ptr, sizeof(st), hipMemcpyDeviceToHost, engine->stream), "moments");
         Check(hipStreamSynchronize(engine->stream), "moments");
         std::memcpy(moments, st.moments, sizeof(st.moments));
     });
 }
 
+int swmi_get_progress(swmi_engine* engine, uint32_t lattice, uint32_t* sweeps_done, uint32_t* n_samples)
+{
+    return Guard([&] {
+        Lattice& l = Eng(engine)->At(lattice);
+        LatState st{};
+        Check(hipMemcpyAsync(&st, l.state.ptr, sizeof(st), hipMemcpyDeviceToHost, engine->stream), "progress");
+        Check(hipStreamSynchronize(engine->stream), "progress");
+        if (sweeps_done)
+            *sweeps_done = st.iteration / 2u;
+        if (n_samples)
+            *n_samples = st.n_samples;
+    });
+}
+
 int swmi_get_series(swmi_engine* engine, uint32_t lattice, int64_t* energy, int64_t* magnetization, uint32_t max, uint32_t* count)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        Lattice& l = engine->At(lattice);
-        swmi::LatState st{};
+        Lattice& l = Eng(engine)->At(lattice);
+        LatState st{};
         Check(hipMemcpyAsync(&st, l.state.ptr, sizeof(st), hipMemcpyDeviceToHost, engine->stream), "series");
         Check(hipStreamSynchronize(engine->stream), "series");
         const uint32_t stored = std::min(st.n_samples, st.series_cap);
@@ -559,8 +717,7 @@ int swmi_get_series(swmi_engine* engine, uint32_t lattice, int64_t* energy, int6
         if (n == 0)
             return;
         std::vector<int64_t> buf(2ull * n);
-        Check(hipMemcpyAsync(buf.data(), l.state.ptr + swmi::kLatStateWords, buf.size() * 8, hipMemcpyDeviceToHost, engine->stream),
-              "series");
+        Check(hipMemcpyAsync(buf.data(), l.state.ptr + kLatStateWords, buf.size() * 8, hipMemcpyDeviceToHost, engine->stream), "series");
         Check(hipStreamSynchronize(engine->stream), "series");
         for (uint32_t i = 0; i < n; ++i)
         {
@@ -575,13 +732,13 @@ int swmi_get_series(swmi_engine* engine, uint32_t lattice, int64_t* energy, int6
 int swmi_reset_measurements(swmi_engine* engine)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        for (auto& l : engine->lattices)
+        for (auto& l : Eng(engine)->lattices)
         {
-            // n_samples (word 1) and the six FP64 moments (bytes 16..63)
-            Check(hipMemsetAsync(reinterpret_cast<uint8_t*>(l->state.ptr) + 4, 0, 4, engine->stream), "reset");
-            Check(hipMemsetAsync(reinterpret_cast<uint8_t*>(l->state.ptr) + 16, 0, 48, engine->stream), "reset");
+            if (!l)
+                continue;
+            Check(hipMemsetAsync(reinterpret_cast<uint8_t*>(l->state.ptr) + 4 * kStSamples, 0, 4, engine->stream), "reset");
+            Check(hipMemsetAsync(reinterpret_cast<uint8_t*>(l->state.ptr) + 4 * kStMoments, 0, sizeof(LatState::moments), engine->stream),
+                  "reset");
         }
         Check(hipStreamSynchronize(engine->stream), "reset");
     });
@@ -590,27 +747,28 @@ int swmi_reset_measurements(swmi_engine* engine)
 int swmi_record_bonds(swmi_engine* engine, int enable)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        Check(hipStreamSynchronize(engine->stream), "record bonds");
+        Check(hipStreamSynchronize(Eng(engine)->stream), "record bonds");
         for (auto& l : engine->lattices)
         {
+            if (!l)
+                continue;
             if (enable && !l->bond_dump.ptr)
                 l->bond_dump.allocate(l->n);
             if (!enable)
                 l->bond_dump.reset();
         }
         engine->recording = enable != 0;
-        engine->UploadDescriptors();
+        engine->Rebuild();
+        Check(hipStreamSynchronize(engine->stream), "record bonds");
     });
 }
 
 int swmi_download_bonds(swmi_engine* engine, uint32_t lattice, uint8_t* bonds)
 {
     return Guard([&] {
-        if (!engine || !bonds)
+        if (!bonds)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         if (!l.bond_dump.ptr)
             throw SwmiError(SWMI_ERR_STATE, "bond recording is not enabled");
         Check(hipMemcpyAsync(bonds, l.bond_dump.ptr, l.n, hipMemcpyDeviceToHost, engine->stream), "bond download");
@@ -621,16 +779,16 @@ int swmi_download_bonds(swmi_engine* engine, uint32_t lattice, uint8_t* bonds)
 int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds)
 {
     return Guard([&] {
-        if (!engine || !bonds)
+        if (!bonds)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         if (!l.bond_in.ptr)
             l.bond_in.allocate(l.n);
         Check(hipMemcpyAsync(l.bond_in.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
         engine->OneLattice(l, l.bond_in.ptr);
-        swmi::LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, l.tw == 64, true, engine->stream);
-        swmi::LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, engine->stream);
-        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, swmi::kWriteLabels, engine->stream);
+        LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, true, engine->stream);
+        LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, engine->stream);
+        LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, kWriteLabels, engine->stream);
         engine->LaunchCheck("label launch");
         Check(hipStreamSynchronize(engine->stream), "label");
     });
@@ -639,11 +797,11 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
 int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels)
 {
     return Guard([&] {
-        if (!engine || !labels)
+        if (!labels)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         engine->OneLattice(l, nullptr);
-        swmi::LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.n_blocks, l.twr, swmi::kWriteLabels, engine->stream);
+        LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, kWriteLabels, engine->stream);
         engine->LaunchCheck("resolve launch");
         Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
         Check(hipStreamSynchronize(engine->stream), "label download");
@@ -653,9 +811,7 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
 int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uint64_t* masks, uint32_t* rot, uint32_t* iteration)
 {
     return Guard([&] {
-        if (!engine)
-            throw SwmiError(SWMI_ERR_INVALID, "null engine");
-        Lattice& l = engine->At(lattice);
+        Lattice& l = Eng(engine)->At(lattice);
         if (states)
             Check(hipMemcpyAsync(states, l.rng.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "rng export");
         if (masks)
@@ -671,10 +827,10 @@ int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uin
 int swmi_get_error_flags(swmi_engine* engine, uint32_t lattice, uint32_t* flags)
 {
     return Guard([&] {
-        if (!engine || !flags)
+        if (!flags)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
-        Lattice& l = engine->At(lattice);
-        Check(hipMemcpyAsync(flags, l.state.ptr + 3, 4, hipMemcpyDeviceToHost, engine->stream), "error flags");
+        Lattice& l = Eng(engine)->At(lattice);
+        Check(hipMemcpyAsync(flags, l.state.ptr + kStErr, 4, hipMemcpyDeviceToHost, engine->stream), "error flags");
         Check(hipStreamSynchronize(engine->stream), "error flags");
     });
 }
@@ -682,8 +838,9 @@ int swmi_get_error_flags(swmi_engine* engine, uint32_t lattice, uint32_t* flags)
 int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
 {
     return Guard([&] {
-        if (!engine || !ms || n_sweeps == 0)
+        if (!ms || n_sweeps == 0)
             throw SwmiError(SWMI_ERR_INVALID, "null argument or zero sweeps");
+        Eng(engine);
         hipEvent_t ev[4];
         for (auto& e : ev)
             Check(hipEventCreate(&e), "event create");
@@ -691,12 +848,14 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
         for (uint32_t i = 0; i < n_sweeps; ++i)
         {
             Check(hipEventRecord(ev[0], engine->stream), "event");
-            swmi::LaunchBondLabel(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->max_twr, engine->all_tw64,
-                                  engine->recording, engine->stream);
+            for (auto& k : engine->classes)
+                LaunchBondLabel(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, engine->recording, engine->stream);
             Check(hipEventRecord(ev[1], engine->stream), "event");
-            swmi::LaunchMerge(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->stream);
+            for (auto& k : engine->classes)
+                LaunchMerge(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, engine->stream);
             Check(hipEventRecord(ev[2], engine->stream), "event");
-            swmi::LaunchFlip(engine->d_desc.ptr, engine->d_block_lat.ptr, engine->total_blocks, engine->max_twr, swmi::kFlip | swmi::kAdvance, engine->stream);
+            for (auto& k : engine->classes)
+                LaunchFlip(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, kFlip | kAdvance, engine->stream);
             Check(hipEventRecord(ev[3], engine->stream), "event");
             Check(hipEventSynchronize(ev[3]), "event sync");
             for (int k = 0; k < 3; ++k)
@@ -706,10 +865,49 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
                 total[k] += t;
             }
         }
+        engine->LaunchCheck("timed sweep launch");
         for (auto& e : ev)
             (void)hipEventDestroy(e);
         for (int k = 0; k < 3; ++k)
             ms[k] = static_cast<float>(total[k] / n_sweeps);
+    });
+}
+
+int swmi_autotune(const swmi_lattice_config* config, uint32_t sweeps, uint32_t* best_tile_rows, float* ms_per_sweep)
+{
+    return Guard([&] {
+        if (!config || sweeps == 0)
+            throw SwmiError(SWMI_ERR_INVALID, "null config or zero sweeps");
+        ValidateConfig(*config);
+        constexpr uint32_t kCand[7] = {1, 2, 4, 8, 16, 32, 64};
+        uint32_t best = 0, prev = 0;
+        float best_ms = 0.0f;
+        for (uint32_t i = 0; i < 7; ++i)
+        {
+            if (ms_per_sweep)
+                ms_per_sweep[i] = -1.0f;
+            const uint32_t twr = GeometryOf(config->n0, config->n1, kCand[i]).twr;
+            if (twr == prev)
+                continue;   // clamped to a height already measured
+            prev = twr;
+            swmi_lattice_config c = *config;
+            c.tile_rows = twr;
+            c.max_samples = 0;
+            const float ms = TimeSweeps(c, sweeps);
+            if (ms_per_sweep)
+                ms_per_sweep[i] = ms;
+            if (best == 0 || ms < best_ms)
+            {
+                best = twr;
+                best_ms = ms;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            g_tuned[{config->n0, config->n1}] = best;
+        }
+        if (best_tile_rows)
+            *best_tile_rows = best;
     });
 }
 
@@ -728,10 +926,10 @@ int swmi_copy_calibration(size_t bytes, uint32_t repeats, float* ms)
         hipEvent_t a, b;
         Check(hipEventCreate(&a), "event");
         Check(hipEventCreate(&b), "event");
-        swmi::LaunchCopy(src.ptr, dst.ptr, n16, nullptr);
+        LaunchCopy(src.ptr, dst.ptr, n16, nullptr);
         Check(hipEventRecord(a, nullptr), "event");
         for (uint32_t i = 0; i < repeats; ++i)
-            swmi::LaunchCopy(src.ptr, dst.ptr, n16, nullptr);
+            LaunchCopy(src.ptr, dst.ptr, n16, nullptr);
         Check(hipEventRecord(b, nullptr), "event");
         Check(hipEventSynchronize(b), "event");
         float t = 0;

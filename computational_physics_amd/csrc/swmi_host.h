@@ -1,0 +1,207 @@
+// swmi_host.h -- host-side objects of the engine, shared by the engine (swmi_engine.hip) and the
+// serving loop (swmi_server.cpp). Not part of the C ABI.
+//
+// An engine owns a set of lattices, their device buffers, one HIP stream and the block work lists over
+// them. Lattices are grouped into kernel classes by the k_bond_label specialisation they need (64-wide
+// tiles or not, tile-height bound 8/16/32/64); a sweep is, for every class, one k_bond_label, one k_merge
+// and one k_flip launch over the class's work list, so a batch of lattices of any sizes costs a handful
+// of launches per sweep. Repeated launch sequences are captured once as hipGraphs and replayed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "swmi_internal.h"
+#include "../../include/swmi.h"
+
+namespace swmi
+{
+    struct SwmiError : std::runtime_error
+    {
+        int code;
+        SwmiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+    };
+
+    void Check(hipError_t e, const char* what);
+    void SetLastError(const std::string& m);
+
+    // Runs f; exceptions become a status code and the thread's last error message.
+    template <typename F>
+    int Guard(F&& f)
+    {
+        try
+        {
+            f();
+            return SWMI_OK;
+        }
+        catch (const SwmiError& e)
+        {
+            SetLastError(e.what());
+            return e.code;
+        }
+        catch (const std::exception& e)
+        {
+            SetLastError(e.what());
+            return SWMI_ERR_INVALID;
+        }
+    }
+
+    float BondProbability(float temperature);
+    uint64_t BondThreshold(float p_add);
+    void ValidateConfig(const swmi_lattice_config& c);
+    void RequireGfx950(int device);
+
+    template <typename T>
+    struct DeviceBuffer
+    {
+        T* ptr = nullptr;
+        size_t count = 0;
+        DeviceBuffer() = default;
+        DeviceBuffer(const DeviceBuffer&) = delete;
+        DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+        DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), count(o.count)
+        {
+            o.ptr = nullptr;
+            o.count = 0;
+        }
+        DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+        {
+            if (this != &o)
+            {
+                reset();
+                ptr = o.ptr;
+                count = o.count;
+                o.ptr = nullptr;
+                o.count = 0;
+            }
+            return *this;
+        }
+        ~DeviceBuffer() { reset(); }
+        void allocate(size_t n)
+        {
+            reset();
+            if (n == 0)
+                return;
+            if (hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)) != hipSuccess)
+            {
+                ptr = nullptr;
+                (void)hipGetLastError();
+                throw SwmiError(SWMI_ERR_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+            }
+            count = n;
+        }
+        void reset()
+        {
+            if (ptr)
+                (void)hipFree(ptr);
+            ptr = nullptr;
+            count = 0;
+        }
+        size_t bytes() const { return count * sizeof(T); }
+    };
+
+    // Tile geometry of a lattice (see swmi_internal.h for the meaning of the fields).
+    struct Geometry
+    {
+        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0;
+    };
+    Geometry GeometryOf(uint32_t n0, uint32_t n1, uint32_t tile_rows);
+    // Tile height for tile_rows == 0: the autotuned value for this extent if one was measured in this
+    // process, else 16 wave rows.
+    uint32_t DefaultTileRows(uint32_t n0, uint32_t n1);
+    // k_bond_label specialisation bound for a tile height: 8, 16, 32 or 64.
+    uint32_t TwrClass(uint32_t twr);
+
+    struct Lattice
+    {
+        swmi_lattice_config cfg{};
+        uint64_t n = 0;
+        uint32_t n_groups = 0;
+        Geometry g;
+        float p_add = 0.0f;
+        uint64_t threshold = 0;
+        uint32_t state_words = 0;
+        DeviceBuffer<uint8_t> spins;
+        DeviceBuffer<uint32_t> rng;
+        DeviceBuffer<uint64_t> masks;
+        DeviceBuffer<uint32_t> rot;
+        DeviceBuffer<uint16_t> local;
+        DeviceBuffer<uint32_t> parent;
+        DeviceBuffer<uint64_t> bbits;
+        DeviceBuffer<uint32_t> state;
+        DeviceBuffer<int32_t> partials;
+        DeviceBuffer<uint16_t> edge_list;
+        DeviceBuffer<uint32_t> edge_count;
+        DeviceBuffer<uint8_t> bond_dump;
+        DeviceBuffer<uint8_t> bond_in;
+
+        // Device bytes of a lattice of this configuration (what Allocate takes).
+        static uint64_t ResidentBytes(const swmi_lattice_config& c);
+        void Allocate(const swmi_lattice_config& c);
+        // Hot start, lcg32 initialisation and fresh counters, enqueued on `stream` (no host sync, no
+        // host memory read after return).
+        void Initialize(const swmi_lattice_config& c, hipStream_t stream);
+        bool SameShape(const swmi_lattice_config& c) const;
+        uint64_t Bytes() const;
+    };
+
+    struct KernelClass
+    {
+        bool tw64 = true;
+        uint32_t twr_t = 16;                 // k_bond_label TWR_MAX
+        uint32_t max_twr = 1;
+        std::vector<uint32_t> block_lat;     // block -> lattice slot
+        DeviceBuffer<uint32_t> d_block_lat;
+        uint32_t total_blocks = 0;
+    };
+
+    struct GraphEntry
+    {
+        uint32_t sweeps = 0;       // sweeps per replay
+        int32_t sample = -1;       // -1: no sample; else the SampleMode taken after every `sep` sweeps
+        uint32_t sep = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+}
+
+struct swmi_engine
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<swmi::Lattice>> lattices;
+    std::vector<swmi::LatDesc> descs;
+    swmi::DeviceBuffer<swmi::LatDesc> d_desc;
+    std::vector<swmi::KernelClass> classes;
+    bool recording = false;   // some lattice records its bonds (selects the inspection build of k_bond_label)
+    bool use_graphs = true;
+    std::vector<swmi::GraphEntry> graphs;   // valid for the current work lists only
+    // single-lattice work lists for the inspection hooks
+    swmi::DeviceBuffer<swmi::LatDesc> d_one_desc;
+    swmi::DeviceBuffer<uint32_t> d_zero_blocks;
+
+    swmi_engine();
+    ~swmi_engine();
+    swmi_engine(const swmi_engine&) = delete;
+    swmi_engine& operator=(const swmi_engine&) = delete;
+
+    swmi::Lattice& At(uint32_t i);
+    swmi::LatDesc Desc(const swmi::Lattice& l, uint32_t block_base) const;
+    // Rebuilds descriptors and class work lists after a change of the lattice set, a temperature or the
+    // recording mode, and drops the captured graphs. The uploads are ordered on the engine stream; the
+    // host copies stay alive until the next rebuild.
+    void Rebuild();
+    void DropGraphs();
+    void LaunchCheck(const char* what);
+    void EnqueueSweeps(uint32_t n);
+    void EnqueueSample(uint32_t mode);
+    // n sweeps (with a sample of `sample_mode` after every `sep` sweeps when sample_mode >= 0), through
+    // captured graphs when enabled.
+    void Run(uint32_t n, uint32_t sep, int32_t sample_mode);
+    void OneLattice(const swmi::Lattice& l, const uint8_t* bond_in);
+    uint64_t TotalSites() const;
+};

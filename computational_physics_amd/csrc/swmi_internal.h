@@ -22,6 +22,7 @@
 // minimum: site(q) = block_base + (q >> 6)*stride + (q & 63), stride = 64*tiles_x.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace swmi
@@ -64,17 +65,37 @@ namespace swmi
         uint32_t flip_rule;
     };
 
-    // Per-lattice device counters and FP64 moments (uint32 words; moments start at word 4).
+    // Per-lattice device counters and FP64 moments (uint32 words; moments start at word kStMoments).
+    // Sweep schedule: k_bond_label advances a lattice only while iteration < it_end (two lcg32 updates per
+    // sweep) and records the decision in `active`, which k_merge and k_flip of the same sweep read (the
+    // iteration itself changes inside k_flip). Conditional samples (sample_mode kSampleDue) are taken
+    // when iteration > last_sample, iteration > it_meas and (iteration - it_meas) % sep2 == 0, i.e. after
+    // every n_sep-th sweep past the warmup of the lattice's own schedule.
     struct LatState
     {
         uint32_t iteration;      // lcg32 Update() count, the same for every group of the lattice
         uint32_t n_samples;      // samples taken
         uint32_t series_cap;     // capacity of the (E, M) series
         uint32_t error_flags;    // set by kernels on a violated invariant (label out of range)
+        uint32_t it_end;         // iteration at which the lattice stops (0xFFFFFFFF: never)
+        uint32_t active;         // 1 if the current sweep advances the lattice
+        uint32_t it_meas;        // iteration at the end of the warmup
+        uint32_t sep2;           // 2 * n_sep
+        uint32_t last_sample;    // iteration of the last sample (0: none)
+        uint32_t pad[3];
         double moments[6];       // samples, sum e, sum e^2, sum |m|, sum m^2, sum m^4
         // followed by int64 series[2 * series_cap]
     };
     constexpr uint32_t kLatStateWords = sizeof(LatState) / 4;
+    constexpr uint32_t kStIteration = 0, kStSamples = 1, kStErr = 3, kStItEnd = 4, kStActive = 5, kStItMeas = 6,
+                       kStSep2 = 7, kStLastSample = 8, kStMoments = 12;
+    static_assert(offsetof(LatState, moments) == 4 * kStMoments, "LatState layout");
+
+    enum SampleMode : uint32_t
+    {
+        kSampleAlways = 0u,  // every lattice, now
+        kSampleDue = 1u      // only lattices whose schedule calls for a sample at this iteration
+    };
 
     enum KernelFlags : uint32_t
     {
@@ -93,8 +114,12 @@ namespace swmi
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
     void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
                     uint32_t flags, hipStream_t stream);
-    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
-    void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, hipStream_t stream);
+    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t sample_mode,
+                           hipStream_t stream);
+    void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, uint32_t sample_mode, hipStream_t stream);
+    void LaunchInitState(uint32_t* state, uint32_t words, uint32_t series_cap, uint32_t it_end, uint32_t it_meas, uint32_t sep2,
+                         hipStream_t stream);
+    void LaunchSetSchedule(uint32_t* state, uint32_t sweeps, uint32_t warmup, uint32_t n_sep, hipStream_t stream);
     void LaunchInitRng(uint8_t* spins, uint32_t* rng, uint64_t* masks, uint32_t* rot, uint32_t n_groups, uint32_t seed,
                        hipStream_t stream);
     void LaunchSetSpins(uint8_t* spins, const uint8_t* src, uint64_t n, hipStream_t stream);

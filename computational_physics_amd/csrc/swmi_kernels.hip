@@ -299,7 +299,15 @@ namespace swmi
         const bool rbound = lx == tw - 1u;
         const bool injected = DBG && d.bond_in != nullptr;
         const uint32_t thr = d.threshold;
-        const uint32_t it1 = (injected ? 0u : Glob(d.state)[0]) + 1u;
+        // sweep schedule: a lattice past its last sweep is left alone by the whole sweep
+        const gptr<uint32_t> stp = Glob(d.state);
+        const uint32_t it0 = injected ? 0u : stp[kStIteration];
+        const bool act = injected || it0 < stp[kStItEnd];
+        if (bl.lb == 0 && threadIdx.x == 0)
+            stp[kStActive] = act ? 1u : 0u;
+        if (!act)
+            return;
+        const uint32_t it1 = it0 + 1u;
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
         const bool rot2 = !injected && (it2 % 15u == 0u);
@@ -596,7 +604,7 @@ namespace swmi
             bb[1u + w] = right_bits;
             Glob(d.edge_count)[kWavesPerBlock * bl.lb + w] = min(nedge, cap);
             if (nedge > cap)
-                (void)__hip_atomic_fetch_or(Glob(d.state) + 3, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 
@@ -635,13 +643,15 @@ namespace swmi
         if (b >= total_blocks)
             return;
         const LatDesc& d = descs[block_lat[b]];
+        if (Glob(d.state)[kStActive] == 0u)
+            return;
         const Block bl = DecodeBlock(d, b);
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         const uint32_t lane = threadIdx.x & 63u;
         const uint32_t w = Uniform(threadIdx.x >> 6);
         const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw, twr = d.twr;
         const uint32_t lx = lane & (tw - 1u);
-        const gptr<uint32_t> err = Glob(d.state) + 3;
+        const gptr<uint32_t> err = Glob(d.state) + kStErr;
 
         // right edge of tile w: lane k = lattice row k of the tile
         const uint64_t right_bits = bb[1u + w];
@@ -685,12 +695,14 @@ namespace swmi
         if (b >= total_blocks)
             return;
         const LatDesc& d = descs[block_lat[b]];
+        const bool do_flip = (flags & kFlip) != 0u;
+        if (do_flip && Glob(d.state)[kStActive] == 0u)
+            return;
         const Block bl = DecodeBlock(d, b);
         const gptr<uint32_t> L = Glob(d.parent);
         const gptr<uint16_t> local = Glob(d.local);
         const gptr<uint8_t> spins = Glob(d.spins);
         const uint32_t qmax = bl.nwr * 64u;
-        const bool do_flip = (flags & kFlip) != 0u;
         const bool write_labels = (flags & kWriteLabels) != 0u;
         const uint32_t sub = threadIdx.x >> 4;
         const uint32_t col = (threadIdx.x & 15u) * 4u;
@@ -789,21 +801,31 @@ namespace swmi
             load(r0);
         }
         if (!ok)
-            (void)__hip_atomic_fetch_or(Glob(d.state) + 3, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            (void)__hip_atomic_fetch_or(Glob(d.state) + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((flags & kAdvance) && bl.lb == 0 && threadIdx.x == 0)
-            Glob(d.state)[0] += 2u;
+            Glob(d.state)[kStIteration] += 2u;
     }
 
     // =============================================================================================
     // Observables: per-tile partial sums of sum s(s_right + s_down) and sum s (lattice.cpp:29-67)
     // =============================================================================================
+    // A conditional sample is due when the lattice has advanced since its last sample, is past its warmup
+    // and sits on its own n_sep grid.
+    __device__ __forceinline__ bool SampleDue(const uint32_t* st)
+    {
+        const uint32_t it = st[kStIteration], meas = st[kStItMeas], sep2 = st[kStSep2];
+        return it > st[kStLastSample] && it > meas && sep2 != 0u && (it - meas) % sep2 == 0u;
+    }
+
     __global__ __launch_bounds__(256) void k_observables(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
-                                                         const uint32_t total_blocks)
+                                                         const uint32_t total_blocks, const uint32_t sample_mode)
     {
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
         const LatDesc& d = descs[block_lat[b]];
+        if (sample_mode == kSampleDue && !SampleDue(d.state))
+            return;
         const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
         const uint32_t w = Uniform(threadIdx.x >> 6);
@@ -831,10 +853,12 @@ namespace swmi
 
     // One workgroup per lattice: reduce the tile partials in a fixed order, append (E, M) to the series
     // and accumulate the FP64 moments (the only FP64 on the device).
-    __global__ __launch_bounds__(256) void k_finalize_samples(const LatDesc* __restrict__ descs)
+    __global__ __launch_bounds__(256) void k_finalize_samples(const LatDesc* __restrict__ descs, const uint32_t sample_mode)
     {
         __shared__ long long red_e[256], red_m[256];
         const LatDesc& d = descs[blockIdx.x];
+        if (sample_mode == kSampleDue && !SampleDue(d.state))
+            return;
         long long e = 0, m = 0;
         for (uint32_t i = threadIdx.x; i < kWavesPerBlock * d.n_blocks; i += 256u)
         {
@@ -872,7 +896,42 @@ namespace swmi
                 series[2u * st->n_samples + 1u] = red_m[0];
             }
             st->n_samples += 1u;
+            st->last_sample = st->iteration;
         }
+    }
+
+    // Fresh per-lattice state: counters and moments zero, no sweep limit unless it_end says otherwise.
+    __global__ __launch_bounds__(256) void k_init_state(uint32_t* __restrict__ state, const uint32_t words, const uint32_t series_cap,
+                                                        const uint32_t it_end, const uint32_t it_meas, const uint32_t sep2)
+    {
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x)
+        {
+            uint32_t v = 0;
+            if (i == 2u)
+                v = series_cap;
+            else if (i == kStItEnd)
+                v = it_end;
+            else if (i == kStActive)
+                v = 1u;
+            else if (i == kStItMeas)
+                v = it_meas;
+            else if (i == kStSep2)
+                v = sep2;
+            state[i] = v;
+        }
+    }
+
+    // Schedule relative to the current iteration: `sweeps` more sweeps (0: no limit), the first `warmup` of
+    // them unsampled, then a conditional sample after every n_sep-th.
+    __global__ void k_set_schedule(uint32_t* __restrict__ state, const uint32_t sweeps, const uint32_t warmup, const uint32_t n_sep)
+    {
+        if (threadIdx.x != 0)
+            return;
+        const uint32_t it = state[kStIteration];
+        state[kStItEnd] = sweeps ? it + 2u * sweeps : 0xFFFFFFFFu;
+        state[kStItMeas] = it + 2u * warmup;
+        state[kStSep2] = 2u * n_sep;
+        state[kStLastSample] = it;
     }
 
     // =============================================================================================
@@ -1026,14 +1085,26 @@ namespace swmi
         k_flip<<<XcdGrid(total_blocks), 256, lds, stream>>>(d_desc, d_block_lat, total_blocks, flags);
     }
 
-    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
+    void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t sample_mode,
+                           hipStream_t stream)
     {
-        k_observables<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+        k_observables<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks, sample_mode);
     }
 
-    void LaunchFinalizeSamples(const LatDesc* d_desc, const uint32_t n_lattices, hipStream_t stream)
+    void LaunchFinalizeSamples(const LatDesc* d_desc, const uint32_t n_lattices, const uint32_t sample_mode, hipStream_t stream)
     {
-        k_finalize_samples<<<n_lattices, 256, 0, stream>>>(d_desc);
+        k_finalize_samples<<<n_lattices, 256, 0, stream>>>(d_desc, sample_mode);
+    }
+
+    void LaunchInitState(uint32_t* state, const uint32_t words, const uint32_t series_cap, const uint32_t it_end, const uint32_t it_meas,
+                         const uint32_t sep2, hipStream_t stream)
+    {
+        k_init_state<<<1, 256, 0, stream>>>(state, words, series_cap, it_end, it_meas, sep2);
+    }
+
+    void LaunchSetSchedule(uint32_t* state, const uint32_t sweeps, const uint32_t warmup, const uint32_t n_sep, hipStream_t stream)
+    {
+        k_set_schedule<<<1, 64, 0, stream>>>(state, sweeps, warmup, n_sep);
     }
 
     void LaunchInitRng(uint8_t* spins, uint32_t* rng, uint64_t* masks, uint32_t* rot, const uint32_t n_groups, const uint32_t seed,

@@ -97,6 +97,27 @@ int swmi_get_threshold(const swmi_engine* engine, uint32_t lattice, float* p_add
  * LatticeMonteCarloAlgorithm::Update (ising/include/lattice_mc.hpp:18) as implemented by
  * SwendsenWang_2D::Update (ising/src/swendsen_wang.cpp:94-112). Asynchronous on the engine stream. ---- */
 int swmi_sweep(swmi_engine* engine, uint32_t n_sweeps);
+/* Launch mode: captured hipGraphs of 16 sweeps (or of sample groups) replayed for long sweep loops
+ * (enable = 1, the default) or every kernel launched one by one (0). Results are identical. The
+ * reference launches every kernel and synchronises after each (swendsen_wang_amd_gpu.cpp:253-492). */
+int swmi_set_graphs(swmi_engine* engine, int enable);
+/* Tile height (in 64-site wave rows) the lattice runs with. */
+int swmi_get_tile_rows(const swmi_engine* engine, uint32_t lattice, uint32_t* tile_rows);
+/* Tile-height autotuner: times plain sweeps of this configuration for each tile height in
+ * {1, 2, 4, 8, 16, 32, 64} (clamped to the lattice; ms_per_sweep[i] = -1 for a height clamped to one already
+ * measured), returns the fastest and remembers it for this extent, so that later lattices created with
+ * tile_rows = 0 use it. Spins, labels and random streams do not depend on the tile height. The
+ * reference fixes the tile from the wavefront size (ising/src/swendsen_wang.cpp:16-31). */
+int swmi_autotune(const swmi_lattice_config* config, uint32_t sweeps, uint32_t* best_tile_rows, float* ms_per_sweep);
+/* Per-lattice schedule, relative to the lattice's current sweep count: it advances `sweeps` more sweeps
+ * (0: no limit) and then stays unchanged while the other lattices of the engine go on; swmi_run_scheduled
+ * samples it after every n_sep-th sweep past the first `warmup`. The measurement protocol is that of
+ * ising/src/ising_2d.cpp:122-148 (warmup, then a sample every N_Sep sweeps), per lattice. */
+int swmi_set_schedule(swmi_engine* engine, uint32_t lattice, uint32_t sweeps, uint32_t warmup, uint32_t n_sep);
+/* n_sweeps sweeps of the engine with the scheduled samples (every lattice on its own schedule). */
+int swmi_run_scheduled(swmi_engine* engine, uint32_t n_sweeps, uint32_t n_sep);
+/* Sweeps done by the lattice since it was created, and samples taken. */
+int swmi_get_progress(swmi_engine* engine, uint32_t lattice, uint32_t* sweeps_done, uint32_t* n_samples);
 
 /* ---- observables. swmi_sample appends one sample (E, M of the current configuration) per lattice to
  * its on-device series and FP64 moments; swmi_run runs n_sweeps sweeps with a sample after every
@@ -137,6 +158,69 @@ int swmi_get_error_flags(swmi_engine* engine, uint32_t lattice, uint32_t* flags)
 int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms);
 /* Device-to-device copy of `bytes` bytes (calibration kernel for HBM byte counters). */
 int swmi_copy_calibration(size_t bytes, uint32_t repeats, float* ms);
+
+/* ---- serving: continuous batching of independent jobs. No counterpart in the reference, which runs one
+ * lattice per process (ising/include/lattice.hpp:56); each job follows the reference's measurement
+ * protocol (ising/src/ising_2d.cpp:122-148): `warmup` sweeps, then `sweeps` sweeps with a sample after
+ * every n_sep-th, FP64 moments as swmi_get_moments. A worker thread owns the GPU: jobs are admitted as
+ * device memory (memory_budget) and slots (max_jobs) free up, run in size-class lanes (one HIP stream
+ * each) in chunks of chunk_sweeps sweeps together with the other resident jobs, and retire after
+ * exactly warmup + sweeps sweeps. A job's result is the same as when it runs alone. ---- */
+typedef struct swmi_server swmi_server;
+
+typedef struct
+{
+    uint64_t memory_budget;  /* device bytes for resident jobs and pooled buffers (0: 200 GiB) */
+    uint32_t max_jobs;       /* most resident jobs (0: 4096) */
+    uint32_t chunk_sweeps;   /* sweeps per scheduling step of a lane, a multiple of n_sep (0: 50) */
+    uint32_t n_sep;          /* sweeps between samples of every job (0: 10) */
+    uint32_t n_lanes;        /* size-class lanes, 1..4 (0: 4: <= 128^2, <= 512^2, <= 2048^2, larger) */
+} swmi_server_config;
+
+typedef struct
+{
+    uint32_t n0, n1;         /* extents, as swmi_lattice_config */
+    float temperature;
+    uint32_t seed;           /* lcg32 seed */
+    uint32_t spin_seed;      /* hot start seed */
+    uint32_t warmup;         /* unsampled sweeps first, a multiple of n_sep */
+    uint32_t sweeps;         /* measured sweeps after the warmup (> 0) */
+    uint64_t tag;            /* caller's value, returned with the result */
+} swmi_job;
+
+typedef struct
+{
+    uint64_t job_id, tag;
+    uint32_t n0, n1;
+    float temperature;
+    uint32_t sweeps_done;    /* warmup + sweeps */
+    uint32_t error_flags;    /* as swmi_get_error_flags */
+    uint32_t status;         /* SWMI_OK, or the code that ended the job */
+    double moments[6];       /* as swmi_get_moments, over the samples after the warmup */
+    int64_t energy;          /* E and M of the final configuration */
+    int64_t magnetization;
+    double t_submit, t_admit, t_finish;   /* seconds on the server clock */
+} swmi_job_result;
+
+typedef struct
+{
+    uint64_t submitted, admitted, finished;
+    uint64_t queued, active;
+    uint64_t bytes_in_use, bytes_pooled;
+    uint64_t spin_updates;   /* sum over finished jobs of (warmup + sweeps) * n0 * n1 */
+    double clock;            /* server clock now, seconds since creation */
+} swmi_server_stats;
+
+int swmi_server_create(const swmi_server_config* config, swmi_server** out);
+/* Finishes the chunks in flight, drops queued jobs and frees the device memory. */
+int swmi_server_destroy(swmi_server* server);
+/* Thread-safe; returns immediately. */
+int swmi_server_submit(swmi_server* server, const swmi_job* job, uint64_t* job_id);
+/* Takes up to max finished results (in completion order). */
+int swmi_server_poll(swmi_server* server, swmi_job_result* out, uint32_t max, uint32_t* count);
+/* Blocks until every submitted job has finished or timeout_ms has passed. */
+int swmi_server_wait(swmi_server* server, uint32_t timeout_ms, uint32_t* unfinished);
+int swmi_server_get_stats(swmi_server* server, swmi_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,188 @@
+"""Batched, graph-captured, scheduled and served runs give the same per-lattice results as the plain path.
+
+Spins and lcg32 streams are per lattice and labels are canonical, so a lattice's trajectory does not depend
+on which other lattices share its launches, on the tile height, on graph capture or on the chunking of
+the serving loop. These tests check that bit for bit, against lone runs on the GPU and against the
+oracle's restatement of the engine's per-site sweep.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TC = 2.2691853
+
+
+def _cfg(swmi, n0, n1, T=TC, seed=0, spin_seed=1, tile_rows=0, max_samples=0):
+    return swmi.LatticeConfig(n0, n1, T, seed=seed, spin_seed=spin_seed, tile_rows=tile_rows, max_samples=max_samples)
+
+
+MIXED = [(32, 32, 2.9, 4), (64, 64, 2.0, 8), (128, 128, TC, 16), (256, 256, 1.8, 32), (16, 64, 2.5, 2),
+         (512, 512, 2.6, 64), (64, 128, TC, 1), (1024, 1024, 2.4, 0)]
+
+
+def test_graph_replays_equal_eager(swmi):
+    """Graph replays (16-sweep units, and sample groups) with eager work and syncs in between, each checked."""
+    cfgs = [_cfg(swmi, n0, n1, T, seed=i + 1, spin_seed=i + 2, tile_rows=tr, max_samples=64)
+            for i, (n0, n1, T, tr) in enumerate(MIXED[:5])]
+    print("graph vs eager", flush=True)
+    with swmi.Engine(cfgs) as g, swmi.Engine(cfgs) as e:
+        e.set_graphs(False)
+        for rep in range(3):
+            g.sweep(32)
+            e.sweep(32)
+            for i in range(len(cfgs)):
+                np.testing.assert_array_equal(g.spins(i), e.spins(i), err_msg=f"replay {rep} lattice {i}")
+                assert g.energy_magnetization(i) == e.energy_magnetization(i)
+            g.synchronize()
+        for rep in range(2):
+            g.run(130, 10)   # 13 groups: two 6-group graph replays and one eager group
+            e.run(130, 10)
+            for i in range(len(cfgs)):
+                np.testing.assert_array_equal(g.moments(i), e.moments(i), err_msg=f"run {rep} lattice {i}")
+                ge, gm = g.series(i)
+                ee, em = e.series(i)
+                np.testing.assert_array_equal(ge, ee)
+                np.testing.assert_array_equal(gm, em)
+        for i in range(len(cfgs)):
+            assert g.rng_state(i)["iteration"] == e.rng_state(i)["iteration"] == 2 * (96 + 260)
+            assert g.error_flags(i) == 0
+
+
+def test_mixed_batch_equals_alone_and_oracle(swmi, oracle):
+    """Lattices of every kernel class (narrow and 64-wide tiles, tile heights 1..64) in one engine."""
+    cfgs = [_cfg(swmi, n0, n1, T, seed=10 + i, spin_seed=20 + i, tile_rows=tr) for i, (n0, n1, T, tr) in enumerate(MIXED)]
+    print("mixed batch", flush=True)
+    with swmi.Engine(cfgs) as batch:
+        batch.sweep(40)
+        got = [batch.spins(i) for i in range(len(cfgs))]
+        assert all(batch.error_flags(i) == 0 for i in range(len(cfgs)))
+    for i, c in enumerate(cfgs):
+        with swmi.Engine(_cfg(swmi, c.n0, c.n1, c.temperature, seed=c.seed, spin_seed=c.spin_seed)) as alone:
+            alone.sweep(40)
+            np.testing.assert_array_equal(alone.spins(), got[i], err_msg=f"lattice {i} ({c.n0}x{c.n1})")
+        if c.n0 * c.n1 <= 128 * 128:
+            ref = oracle.PerSite(c.n0, c.n1, c.temperature, seed=c.seed, spin_seed=c.spin_seed)
+            for _ in range(40):
+                ref.sweep()
+            np.testing.assert_array_equal(ref.spins, got[i], err_msg=f"oracle, lattice {i}")
+
+
+def test_schedule_stops_one_lattice(swmi):
+    cfgs = [_cfg(swmi, 128, 128, seed=1), _cfg(swmi, 64, 64, 2.0, seed=2)]
+    with swmi.Engine(cfgs) as e:
+        e.set_schedule(0, sweeps=7)
+        e.sweep(20)
+        assert e.progress(0)[0] == 7 and e.progress(1)[0] == 20
+        s0 = e.spins(0)
+    with swmi.Engine(cfgs[0]) as alone:
+        alone.sweep(7)
+        np.testing.assert_array_equal(alone.spins(), s0)
+
+
+@pytest.mark.parametrize("sweeps,warmup,extra", [(100, 20, 0), (100, 30, 47), (60, 0, 13)])
+def test_scheduled_samples_equal_plain_protocol(swmi, sweeps, warmup, extra):
+    """warmup + sweeps with a sample every 10th, run inside a longer loop, equals the plain protocol."""
+    c = _cfg(swmi, 128, 64, 2.1, seed=5, spin_seed=6, max_samples=32)
+    with swmi.Engine([c, _cfg(swmi, 64, 64, seed=7)]) as e:
+        e.set_schedule(0, sweeps=warmup + sweeps, warmup=warmup, n_sep=10)
+        e.run_scheduled(warmup + sweeps + extra, 10)
+        mom_s = e.moments(0)
+        ser_s = e.series(0)
+        assert e.progress(0) == (warmup + sweeps, sweeps // 10)
+    with swmi.Engine(c) as p:
+        p.sweep(warmup)
+        p.reset_measurements()
+        p.run(sweeps, 10)
+        np.testing.assert_array_equal(p.moments(0), mom_s)
+        np.testing.assert_array_equal(p.series(0)[0], ser_s[0])
+
+
+def test_autotune_picks_a_height_and_results_do_not_depend_on_it(swmi):
+    c = _cfg(swmi, 256, 256, seed=3)
+    best, ms = swmi.autotune(c, sweeps=10)
+    print(f"autotune 256^2: best {best}, ms {ms}", flush=True)
+    assert best in (1, 2, 4, 8, 16, 32, 64)
+    assert min(m for m in ms if m > 0) == pytest.approx(ms[[1, 2, 4, 8, 16, 32, 64].index(best)])
+    with swmi.Engine(c) as tuned, swmi.Engine(_cfg(swmi, 256, 256, seed=3, tile_rows=16 if best != 16 else 4)) as other:
+        assert tuned.tile_rows() == best
+        tuned.sweep(20)
+        other.sweep(20)
+        np.testing.assert_array_equal(tuned.spins(), other.spins())
+
+
+def _plain_job_result(swmi, job):
+    """The same job through the plain path: warmup, reset, then samples every 10 sweeps."""
+    c = _cfg(swmi, job.n0, job.n1, job.temperature, seed=job.seed, spin_seed=job.spin_seed)
+    with swmi.Engine(c) as p:
+        p.sweep(job.warmup)
+        p.reset_measurements()
+        p.run(job.sweeps, 10)
+        return p.moments(0), p.energy_magnetization(0)
+
+
+def _serve(swmi, jobs, delays=None, **kw):
+    with swmi.Server(**kw) as srv:
+        ids = {}
+        for k, j in enumerate(jobs):
+            if delays is not None:
+                time.sleep(delays[k])
+            ids[srv.submit(j)] = k
+        assert srv.wait(120000) == 0
+        res = srv.poll(len(jobs) + 8)
+        st = srv.stats()
+    assert len(res) == len(jobs)
+    out = {ids[r["job_id"]]: r for r in res}
+    for k, r in out.items():
+        assert r["status"] == 0 and r["error_flags"] == 0, r
+        assert r["sweeps_done"] == jobs[k].warmup + jobs[k].sweeps
+        assert r["t_submit"] <= r["t_admit"] <= r["t_finish"]
+    return out, st
+
+
+def test_server_job_alone_equals_in_mix_equals_plain(swmi):
+    target = swmi.Job(128, 128, TC, seed=9, spin_seed=4, warmup=50, sweeps=300, tag=1)
+    alone, _ = _serve(swmi, [target])
+    mix_jobs = [swmi.Job(64, 64, 2.0, seed=1, warmup=0, sweeps=700),
+                swmi.Job(1024, 1024, 2.5, seed=2, warmup=20, sweeps=90),
+                target,
+                swmi.Job(32, 32, 3.1, seed=3, warmup=10, sweeps=1234),
+                swmi.Job(256, 512, 1.7, seed=4, warmup=0, sweeps=333),
+                swmi.Job(128, 128, TC, seed=9, spin_seed=4, warmup=50, sweeps=300, tag=2),
+                swmi.Job(2048, 2048, TC, seed=5, warmup=0, sweeps=40)]
+    print("server mix", flush=True)
+    mixed, st = _serve(swmi, mix_jobs, delays=[0, 0.002, 0.01, 0, 0.003, 0.02, 0])
+    assert st["finished"] == len(mix_jobs) and st["active"] == 0 and st["queued"] == 0
+    a = alone[0]
+    for k in (2, 5):
+        m = mixed[k]
+        np.testing.assert_array_equal(m["moments"], a["moments"], err_msg=f"job {k}")
+        assert (m["energy"], m["magnetization"]) == (a["energy"], a["magnetization"])
+    mom, em = _plain_job_result(swmi, target)
+    np.testing.assert_array_equal(a["moments"], mom)
+    assert (a["energy"], a["magnetization"]) == em
+    # the other jobs of the mix against the plain path too (sweeps a multiple of 10 there)
+    for k in (0, 1, 4):
+        j = mix_jobs[k]
+        if j.sweeps % 10 == 0:
+            mom, em = _plain_job_result(swmi, j)
+            np.testing.assert_array_equal(mixed[k]["moments"], mom, err_msg=f"job {k}")
+            assert (mixed[k]["energy"], mixed[k]["magnetization"]) == em
+
+
+def test_server_admits_under_a_small_memory_budget(swmi):
+    """A budget for about two 512^2 jobs: the rest wait in the queue and are admitted as memory frees up."""
+    jobs = [swmi.Job(512, 512, 2.0 + 0.1 * k, seed=k, warmup=0, sweeps=60 + 10 * k) for k in range(6)]
+    budget = 2 * 512 * 512 * 12 + 200000
+    out, st = _serve(swmi, jobs, memory_budget=budget, n_lanes=1)
+    admits = sorted(r["t_admit"] for r in out.values())
+    finishes = sorted(r["t_finish"] for r in out.values())
+    # at most two resident at once: the third admission comes after the first completion
+    assert admits[2] >= finishes[0]
+    assert st["bytes_in_use"] <= budget
+    for k in (0, 3):
+        mom, em = _plain_job_result(swmi, jobs[k])
+        np.testing.assert_array_equal(out[k]["moments"], mom)
+        assert (out[k]["energy"], out[k]["magnetization"]) == em
