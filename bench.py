@@ -7,6 +7,16 @@ HBM. A "step" is one sweep (bond activation + labelling + merge + flip of all 16
 spin-updates/s = steps * N / seconds between two device synchronisations.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 4096] [--temperature T] [--no-cpu-baseline]
+                    [--no-batched] [--no-served]
+
+Two more workloads are reported in the same JSON line:
+  batched  256 independent lattices, L = 2^k with k uniform in {5..10}, T uniform in [1.8, 2.8], one seed
+           each (workload drawn from numpy default_rng(0)), 1000 warmup + 5000 sweeps in one engine (one
+           work list, graph-captured sweep loops, autotuned tile height per L), against the same lattices
+           run one by one through the plain path (sampled: 100 timed sweeps per lattice).
+  served   a Poisson stream of jobs (L = 2^k, k in {6..12}; T uniform in [1.5, 3.5]; sweeps log-uniform in
+           [1e3, 1e5]; a tenth of them warmup) at an offered load of 0.8 of the plain 4096^2 rate, through the
+           continuous-batching server; p50/p99 completion latency and aggregate spin-updates/s.
 
 Roofline: the dominant kernel (bond activation + tile labelling) is HBM-bound; its algorithmic bytes
 are 13 per site per launch (spin read 1, lcg32 state read+write 8, label write 4). The whole sweep moves
@@ -66,6 +76,10 @@ def cpu_baseline(n: int, temperature: float) -> dict | None:
         return None
 
 
+def log(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def load_traffic(n: int) -> float | None:
     """HBM bytes per k_bond_label launch from the committed rocprofv3 PMC summary (scaled FETCH+WRITE)."""
     f = ROOT / "profiles" / f"traffic_bond_label_{n}.json"
@@ -77,6 +91,77 @@ def load_traffic(n: int) -> float | None:
     return None
 
 
+def batched_leg(cpa, warmup: int = 1000, sweeps: int = 5000, alone_sweeps: int = 100) -> dict:
+    import numpy as np
+    rng = np.random.default_rng(0)
+    ls = (2 ** rng.integers(5, 11, size=256)).tolist()
+    ts = rng.uniform(1.8, 2.8, size=256).astype(np.float32).tolist()
+    tuned = {}
+    for L in sorted(set(ls)):
+        tuned[L] = cpa.autotune(cpa.LatticeConfig(L, L, 2.2691853, seed=1), sweeps=10)[0]
+    cfgs = [cpa.LatticeConfig(L, L, t, seed=1000 + i, spin_seed=1 + i) for i, (L, t) in enumerate(zip(ls, ts))]
+    sites = sum(L * L for L in ls)
+    with cpa.Engine(cfgs) as eng:
+        eng.sweep(warmup)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.sweep(sweeps)
+        eng.synchronize()
+        dt = time.perf_counter() - t0
+        flags = max(eng.error_flags(i) for i in range(len(cfgs)))
+    batched = sweeps * sites / dt
+    # the same lattices one by one through the plain path (one engine each; timed sample of the sweeps)
+    t_alone = 0.0
+    for c in cfgs:
+        with cpa.Engine(c) as e1:
+            e1.sweep(10)
+            e1.synchronize()
+            t0 = time.perf_counter()
+            e1.sweep(alone_sweeps)
+            e1.synchronize()
+            t_alone += time.perf_counter() - t0
+    alone = alone_sweeps * sites / t_alone
+    return {"value": batched, "unit": "spin-updates/s", "lattices": len(cfgs), "sites": sites, "warmup": warmup,
+            "sweeps": sweeps, "seconds": dt, "one_by_one": alone, "one_by_one_sample_sweeps": alone_sweeps,
+            "speedup_vs_one_by_one": batched / alone, "tile_rows_by_L": tuned, "error_flags": flags,
+            "workload": "256 lattices, L=2^k k~U{5..10}, T~U[1.8,2.8], default_rng(0); one engine, graph-captured"}
+
+
+def served_leg(cpa, capacity: float, n_jobs: int = 32, load: float = 0.8, seed: int = 1) -> dict:
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(6, 13, size=n_jobs)
+    temps = rng.uniform(1.5, 3.5, size=n_jobs).astype(np.float32)
+    total = np.floor(10 ** rng.uniform(3, 5, size=n_jobs)).astype(np.int64)
+    warm = (total // 10 // 10) * 10
+    work = np.array([(4 ** int(k)) * int(t) for k, t in zip(ks, total)], dtype=np.float64)
+    mean_work = float(np.mean([(4 ** k) for k in range(6, 13)])) * (1e5 - 1e3) / np.log(100.0)
+    rate = load * capacity / mean_work          # jobs per second at the stated offered load
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+    jobs = [cpa.Job(2 ** int(k), 2 ** int(k), float(t), seed=5000 + i, spin_seed=1 + i, warmup=int(w), sweeps=int(n - w),
+                    tag=i) for i, (k, t, n, w) in enumerate(zip(ks, temps, total, warm))]
+    with cpa.Server(memory_budget=200 * 10 ** 9) as srv:
+        t0 = time.perf_counter()
+        for j, a in zip(jobs, arrivals):
+            now = time.perf_counter() - t0
+            if a > now:
+                time.sleep(a - now)
+            srv.submit(j)
+        left = srv.wait(600000)
+        res = srv.poll(n_jobs + 8)
+        st = srv.stats()
+    lat = np.array([r["t_finish"] - r["t_submit"] for r in res])
+    first = min(r["t_submit"] for r in res)
+    last = max(r["t_finish"] for r in res)
+    return {"p50_latency_s": float(np.percentile(lat, 50)), "p90_latency_s": float(np.percentile(lat, 90)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
+            "spin_updates_per_s": float(st["spin_updates"] / (last - first)), "jobs": len(res), "unfinished": left,
+            "failed": sum(1 for r in res if r["status"] != 0), "offered_load": load, "arrival_rate_per_s": rate,
+            "makespan_s": last - first, "work_spin_updates": float(work.sum()),
+            "workload": f"{n_jobs} Poisson jobs, L=2^k k~U{{6..12}}, T~U[1.5,3.5], sweeps~logU[1e3,1e5] (10% warmup), "
+                        f"default_rng({seed}); 200 GB budget, 4 size-class lanes"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +171,9 @@ def main() -> int:
     ap.add_argument("--temperature", type=float, default=T_C)
     ap.add_argument("--kernel-sweeps", type=int, default=100, help="sweeps timed kernel by kernel with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a device copy of BYTES bytes (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -120,7 +208,7 @@ def main() -> int:
     sweep_gbps = SWEEP_BYTES_PER_SITE * sites * args.steps / dt / 1e9
 
     result = {
-        "metric": "spin-updates/s at 4096^2 T_c (plain)",
+        "metric": "spin-updates/s at 4096^2 T_c (plain vs batched mixed-L); p99 served latency",
         "value": value,
         "unit": "spin-updates/s",
         "n_gpus": 1,
@@ -146,8 +234,17 @@ def main() -> int:
                                                           "GBps": 2 * args.copy_calibration / copy_ms / 1e6},
         "error_flags": flags,
     }
+    log(f"plain {n}x{n}: {value:.3e} spin-updates/s, {ms_per_step * 1e3:.1f} us/sweep")
+    if not args.no_batched:
+        result["batched"] = batched_leg(cpa)
+        log(f"batched: {result['batched']['value']:.3e} spin-updates/s, "
+            f"{result['batched']['speedup_vs_one_by_one']:.2f}x one by one")
+    if not args.no_served:
+        result["served"] = served_leg(cpa, capacity=value, n_jobs=args.served_jobs)
+        log(f"served: p50 {result['served']['p50_latency_s']:.3f} s, p99 {result['served']['p99_latency_s']:.3f} s")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, args.temperature)
+        log("cpu baseline done")
     print(json.dumps(result))
     return 0
 
