@@ -127,6 +127,7 @@ namespace swmi
         g.n_blocks = g.tiles_x * g.blocks_y;
         // distinct clusters reaching a block's boundary <= boundary sites (2 columns + 2 rows)
         g.edge_cap = std::min<uint32_t>(g.twr * 64u, 2u * g.tw + 2u * kWavesPerBlock * g.twr * g.rpw);
+        g.ecol_stride = kWavesPerBlock * g.twr * g.rpw;
         return g;
     }
 
@@ -136,7 +137,7 @@ namespace swmi
         const uint64_t n = static_cast<uint64_t>(c.n0) * c.n1, groups = n / 64;
         return n * (1 + 4 + 2 + 4) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
                4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
-               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4);
+               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4) + 4ull * g.ecol_stride * g.n_blocks;
     }
 
     bool Lattice::SameShape(const swmi_lattice_config& c) const
@@ -148,7 +149,8 @@ namespace swmi
     uint64_t Lattice::Bytes() const
     {
         return spins.bytes() + rng.bytes() + masks.bytes() + rot.bytes() + local.bytes() + parent.bytes() + bbits.bytes() +
-               state.bytes() + partials.bytes() + edge_list.bytes() + edge_count.bytes() + bond_dump.bytes() + bond_in.bytes();
+               state.bytes() + partials.bytes() + edge_list.bytes() + edge_count.bytes() + ecol.bytes() + bond_dump.bytes() +
+               bond_in.bytes();
     }
 
     void Lattice::Allocate(const swmi_lattice_config& c)
@@ -169,6 +171,7 @@ namespace swmi
         partials.allocate(2ull * kWavesPerBlock * g.n_blocks);
         edge_list.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
         edge_count.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
+        ecol.allocate(2ull * g.ecol_stride * g.n_blocks);
     }
 
     void Lattice::Initialize(const swmi_lattice_config& c, hipStream_t stream)
@@ -241,6 +244,8 @@ LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
     d.edge_list = l.edge_list.ptr;
     d.edge_count = l.edge_count.ptr;
     d.edge_cap = l.g.edge_cap;
+    d.ecol = l.ecol.ptr;
+    d.ecol_stride = l.g.ecol_stride;
     d.n0 = l.cfg.n0;
     d.n1 = l.cfg.n1;
     d.tw = l.g.tw;

@@ -108,7 +108,7 @@ namespace swmi
     // Tile geometry of a lattice (see swmi_internal.h for the meaning of the fields).
     struct Geometry
     {
-        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0;
+        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0, ecol_stride = 0;
     };
     Geometry GeometryOf(uint32_t n0, uint32_t n1, uint32_t tile_rows);
     // Tile height for tile_rows == 0: the autotuned value for this extent if one was measured in this
@@ -137,6 +137,7 @@ namespace swmi
         DeviceBuffer<int32_t> partials;
         DeviceBuffer<uint16_t> edge_list;
         DeviceBuffer<uint32_t> edge_count;
+        DeviceBuffer<uint16_t> ecol;
         DeviceBuffer<uint8_t> bond_dump;
         DeviceBuffer<uint8_t> bond_in;
 

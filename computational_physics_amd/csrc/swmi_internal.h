@@ -12,6 +12,8 @@
 //   edge_list  uint16[4*blocks*edge_cap]  per tile: block-local indices of the roots of its clusters
 //                            that reach the block edge (the only roots the resolve pass must chase)
 //   edge_count uint32[4*blocks]           entries of each tile's list
+//   ecol     uint16[2*ecol_stride*blocks] labels of each block's first (side 0) and last (side 1) column,
+//                            one entry per lattice row of the block, for the cross-block merge
 //
 // Tiling. A "wave row" is 64 consecutive row-major sites, which is exactly one lcg32 group:
 //   n0 >= 64 (n0 % 64 == 0): a wave row is 64 columns of one lattice row;
@@ -51,6 +53,8 @@ namespace swmi
         int32_t* partials;       // 2 int32 (E, M) per tile for observables
         uint16_t* edge_list;     // kWavesPerBlock lists of edge_cap entries per block
         uint32_t* edge_count;    // kWavesPerBlock counts per block
+        uint16_t* ecol;          // block edge-column labels: [block][side][lattice row in block]
+        uint32_t ecol_stride;    // lattice rows per block: kWavesPerBlock * twr * rpw
         uint32_t edge_cap;       // capacity of one tile's list: min(tile sites, block boundary sites)
         uint32_t n0, n1;
         uint32_t tw;             // tile width in sites: min(n0, 64)

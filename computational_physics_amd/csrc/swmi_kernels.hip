@@ -234,7 +234,7 @@ namespace swmi
     }
 
     // Branch-free union of the pairs (a, b) held by the lanes, all lanes in lockstep. Each step moves
-    // both ends one parent up; when both are roots the larger is linked under the smaller with
+    // both ends up (two parents, halving the path); when both are roots the larger is linked under the smaller with
     // atomicMin, and a lost race continues with the value found (the loop of the reference Merge,
     // swendsen_wang_amd_gpu.cpp:368-401). Lanes with nothing to do hold a == b; for them, and for ends
     // that are not roots yet, the atomic is atomicMin(P[hi], hi), which never changes P (P[x] <= x).
@@ -249,8 +249,16 @@ namespace swmi
             const uint32_t old = __hip_atomic_fetch_min(&P[hi], (roots && a != b) ? lo : hi, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
             const bool linked = roots && (a == b || old == hi);
-            a = roots ? (linked ? hi : lo) : pa;
-            b = roots ? (linked ? hi : old) : pb;
+            // ends that are not roots move two steps up and point their node at the grandparent (path
+            // halving, monotone like every write to P)
+            const uint32_t ga = roots ? 0u : LdsLoad(&P[pa]);
+            const uint32_t gb = roots ? 0u : LdsLoad(&P[pb]);
+            if (!roots && ga != pa)
+                LdsMin(&P[a], ga);
+            if (!roots && gb != pb)
+                LdsMin(&P[b], gb);
+            a = roots ? (linked ? hi : lo) : ga;
+            b = roots ? (linked ? hi : old) : gb;
         }
     }
 
@@ -317,6 +325,7 @@ namespace swmi
         const gptr<uint32_t> rotp = Glob(d.rot);
         const gptr<const uint8_t> bond_in = Glob(d.bond_in);
         const gptr<uint16_t> local = Glob(d.local);
+        const gptr<uint16_t> ecol = Glob(d.ecol);
         const gptr<uint32_t> parent = Glob(d.parent);
 
         const uint32_t r_begin = w * twr;
@@ -584,7 +593,10 @@ namespace swmi
             const uint32_t site = bl.base + (r_begin + r) * bl.stride + lane;
             const uint32_t root = LdsLoad(&P[q]);
             const bool flagged = ((LdsLoad(&F[root >> 5]) >> (root & 31u)) & 1u) != 0u;
-            local[site] = static_cast<uint16_t>(root | (flagged ? kEdgeFlag : 0u));
+            const uint16_t lab = static_cast<uint16_t>(root | (flagged ? kEdgeFlag : 0u));
+            local[site] = lab;
+            if (lx == 0u || rbound)
+                ecol[(2u * bl.lb + (rbound ? 1u : 0u)) * d.ecol_stride + (r_begin + r) * rpw + lrow] = lab;
             const uint64_t K = __ballot(flagged && root == q);
             if (LaneBit(K))
             {
@@ -613,17 +625,18 @@ namespace swmi
     // =============================================================================================
     // One workgroup per block: wavefront w takes the right-edge word of tile w (one lane per lattice row),
     // wavefront 0 also the down edge (one lane per column), so the unions of a block run in parallel.
-    __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t a_site,
-                                              const uint32_t b_site, const uint32_t base_b, const bool same_line,
+    // bit `lane` of `bits`: a bond from the site labelled *pa (this block) to the site labelled *pb (block
+    // at base_b); same_line: lanes lane-1 and lane are neighbours along the edge (runs deduplicate)
+    __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const gptr<const uint16_t> pa,
+                                              const gptr<const uint16_t> pb, const uint32_t base_b, const bool same_line,
                                               gptr<uint32_t> err)
     {
         const uint32_t lane = threadIdx.x & 63u;
         const bool bit = LaneBit(bits);
-        const gptr<uint16_t> local = Glob(d.local);
         uint32_t a = 0, bq = 0;
         if (bit)
         {
-            const uint32_t va = local[a_site], vb = local[b_site];
+            const uint32_t va = *pa, vb = *pb;
             if (!(va & kEdgeFlag) || !(vb & kEdgeFlag))
                 (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a = SiteOf(bl.base, bl.stride, va & 0x7FFFu);
@@ -653,15 +666,16 @@ namespace swmi
         const uint32_t lx = lane & (tw - 1u);
         const gptr<uint32_t> err = Glob(d.state) + kStErr;
 
-        // right edge of tile w: lane k = lattice row k of the tile
+        // right edge of tile w: lane k = lattice row k of the tile; labels from the compact edge columns
         const uint64_t right_bits = bb[1u + w];
         if (right_bits)
         {
             const uint32_t sx_r = (bl.sx + 1u == d.tiles_x) ? 0u : bl.sx + 1u;
-            const uint32_t xr = bl.sx * tw + tw - 1u;
-            const uint32_t xr1 = (xr + 1u == n0) ? 0u : xr + 1u;
-            const uint32_t y = (bl.wyb + w * twr) * rpw + lane;
-            MergeEdge(d, bl, right_bits, y * n0 + xr, y * n0 + xr1, BlockBase(d, sx_r, bl.sy), lane > 0, err);
+            const uint32_t lb_r = bl.sy * d.tiles_x + sx_r;
+            const uint32_t row = w * twr * rpw + lane;
+            const gptr<const uint16_t> ec = Glob(static_cast<const uint16_t*>(d.ecol));
+            MergeEdge(d, bl, right_bits, ec + (2u * bl.lb + 1u) * d.ecol_stride + row, ec + (2u * lb_r) * d.ecol_stride + row,
+                      BlockBase(d, sx_r, bl.sy), lane > 0, err);
         }
         // down edge: lanes of the last lattice row of the block
         const uint64_t down_bits = (w == 0) ? bb[0] : 0ull;
@@ -670,8 +684,9 @@ namespace swmi
             const uint32_t y = (bl.wyb + bl.nwr - 1u) * rpw + lane / tw;
             const uint32_t x = bl.sx * tw + lx;
             const bool wrap = y + 1u == n1;
-            MergeEdge(d, bl, down_bits, y * n0 + x, (wrap ? 0u : y + 1u) * n0 + x, BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u),
-                      lx != 0, err);
+            const gptr<const uint16_t> loc = Glob(static_cast<const uint16_t*>(d.local));
+            MergeEdge(d, bl, down_bits, loc + y * n0 + x, loc + (wrap ? 0u : y + 1u) * n0 + x,
+                      BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u), lx != 0, err);
         }
     }
 
