@@ -22,6 +22,9 @@ import numpy as np
 __all__ = ["SwmiError", "LatticeConfig", "Engine", "Job", "Server", "library_path", "load_library", "device_info",
            "copy_calibration", "autotune", "REFERENCE_TEMPERATURE"]
 
+FLIP_PARITY = 0   # flip a cluster iff its canonical root index is odd (the reference rule)
+FLIP_RANDOM = 1   # flip a cluster on one random bit of its root (MurmurHash3 of root, sweep, seed)
+
 # ising/src/ising_2d.cpp:33, as the float the reference stores it in.
 REFERENCE_TEMPERATURE = float(np.float32(2.2691853142130221))
 
@@ -58,7 +61,7 @@ class _ServerConfig(ctypes.Structure):
 class _Job(ctypes.Structure):
     _fields_ = [("n0", ctypes.c_uint32), ("n1", ctypes.c_uint32), ("temperature", ctypes.c_float),
                 ("seed", ctypes.c_uint32), ("spin_seed", ctypes.c_uint32), ("warmup", ctypes.c_uint32),
-                ("sweeps", ctypes.c_uint32), ("tag", ctypes.c_uint64)]
+                ("sweeps", ctypes.c_uint32), ("flip_rule", ctypes.c_uint32), ("tag", ctypes.c_uint64)]
 
 
 class _JobResult(ctypes.Structure):
@@ -117,6 +120,10 @@ def load_library() -> ctypes.CDLL:
         "swmi_time_kernels": (ctypes.c_int, [vp, u32, P(f32)]),
         "swmi_copy_calibration": (ctypes.c_int, [ctypes.c_size_t, u32, P(f32)]),
         "swmi_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
+        "swmi_checkpoint_size": (ctypes.c_int, [vp, u32, P(u64)]),
+        "swmi_checkpoint": (ctypes.c_int, [vp, u32, vp, u64]),
+        "swmi_restore": (ctypes.c_int, [vp, u32, vp, u64]),
+        "swmi_error_bars": (ctypes.c_int, [vp, u32, u32, P(f64)]),
         "swmi_get_tile_rows": (ctypes.c_int, [vp, u32, P(u32)]),
         "swmi_autotune": (ctypes.c_int, [P(_Config), u32, P(u32), P(f32)]),
         "swmi_set_schedule": (ctypes.c_int, [vp, u32, u32, u32, u32]),
@@ -329,6 +336,26 @@ class Engine:
         _check(self._lib.swmi_rng_export(self._h, i, states.ctypes.data, masks.ctypes.data, rot.ctypes.data, ctypes.byref(it)))
         return {"states": states, "masks": masks, "rot": rot, "iteration": it.value}
 
+    # -- checkpoint / resume --
+    def checkpoint(self, i: int = 0) -> bytes:
+        """The lattice's complete chain state (spins, lcg32 states, counters, moments, series)."""
+        size = ctypes.c_uint64()
+        _check(self._lib.swmi_checkpoint_size(self._h, i, ctypes.byref(size)))
+        buf = ctypes.create_string_buffer(size.value)
+        _check(self._lib.swmi_checkpoint(self._h, i, buf, size.value))
+        return buf.raw
+
+    def restore(self, data: bytes, i: int = 0) -> None:
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(self._lib.swmi_restore(self._h, i, buf, len(data)))
+
+    def error_bars(self, i: int = 0, n_bins: int = 50) -> dict:
+        """Binning / jackknife error bars from the on-device series (needs max_samples > 0)."""
+        out = (ctypes.c_double * 8)()
+        _check(self._lib.swmi_error_bars(self._h, i, n_bins, out))
+        keys = ("e", "e_err", "abs_m", "abs_m_err", "c", "c_err", "tau_int_e", "samples")
+        return dict(zip(keys, out[:]))
+
     def error_flags(self, i: int = 0) -> int:
         f = ctypes.c_uint32()
         _check(self._lib.swmi_get_error_flags(self._h, i, ctypes.byref(f)))
@@ -351,10 +378,11 @@ class Job:
     warmup: int = 0
     sweeps: int = 1000
     tag: int = 0
+    flip_rule: int = 0
 
     def _c(self) -> _Job:
         return _Job(self.n0, self.n1, self.temperature, self.seed & 0xFFFFFFFF, self.spin_seed & 0xFFFFFFFF, self.warmup,
-                    self.sweeps, self.tag)
+                    self.sweeps, self.flip_rule, self.tag)
 
 
 def _result_dict(r: _JobResult) -> dict:

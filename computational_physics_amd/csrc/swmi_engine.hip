@@ -79,7 +79,7 @@ namespace swmi
             bad("lattice too large: n0*n1 must be < 2^31");
         if (!(c.temperature > 0.0f) || !std::isfinite(c.temperature))
             bad("temperature must be positive and finite");
-        if (c.flip_rule != SWMI_FLIP_PARITY)
+        if (c.flip_rule != SWMI_FLIP_PARITY && c.flip_rule != SWMI_FLIP_RANDOM)
             bad("unknown flip rule " + std::to_string(c.flip_rule));
         if (c.max_samples > (1u << 28))
             bad("max_samples too large");
@@ -258,6 +258,7 @@ LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
     d.block_base = block_base;
     d.n_blocks = l.g.n_blocks;
     d.flip_rule = l.cfg.flip_rule;
+    d.seed = l.cfg.seed;
     return d;
 }
 
@@ -826,6 +827,176 @@ int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uin
         if (iteration)
             Check(hipMemcpyAsync(iteration, l.state.ptr, 4, hipMemcpyDeviceToHost, engine->stream), "rng export");
         Check(hipStreamSynchronize(engine->stream), "rng export");
+    });
+}
+
+namespace
+{
+    struct CheckpointHeader
+    {
+        char magic[8];           // "SWMICKP1"
+        uint32_t n0, n1;
+        uint32_t seed, flip_rule;
+        float temperature;
+        uint32_t state_words;
+        uint64_t n;              // sites
+        uint32_t n_groups;
+        uint32_t reserved[5];
+    };
+    static_assert(sizeof(CheckpointHeader) == 64, "checkpoint header is 64 bytes");
+
+    uint64_t CheckpointBytes(const Lattice& l)
+    {
+        return sizeof(CheckpointHeader) + l.n * (1 + 4) + l.n_groups * (3ull * 8 + 4) + 4ull * l.state_words;
+    }
+}
+
+int swmi_checkpoint_size(swmi_engine* engine, uint32_t lattice, uint64_t* bytes)
+{
+    return Guard([&] {
+        if (!bytes)
+            throw SwmiError(SWMI_ERR_INVALID, "null argument");
+        *bytes = CheckpointBytes(Eng(engine)->At(lattice));
+    });
+}
+
+int swmi_checkpoint(swmi_engine* engine, uint32_t lattice, void* buffer, uint64_t bytes)
+{
+    return Guard([&] {
+        Lattice& l = Eng(engine)->At(lattice);
+        if (!buffer || bytes < CheckpointBytes(l))
+            throw SwmiError(SWMI_ERR_INVALID, "checkpoint buffer too small (" + std::to_string(CheckpointBytes(l)) + " bytes needed)");
+        CheckpointHeader h{};
+        std::memcpy(h.magic, "SWMICKP1", 8);
+        h.n0 = l.cfg.n0;
+        h.n1 = l.cfg.n1;
+        h.seed = l.cfg.seed;
+        h.flip_rule = l.cfg.flip_rule;
+        h.temperature = l.cfg.temperature;
+        h.state_words = l.state_words;
+        h.n = l.n;
+        h.n_groups = l.n_groups;
+        uint8_t* out = static_cast<uint8_t*>(buffer);
+        std::memcpy(out, &h, sizeof(h));
+        uint8_t* q = out + sizeof(h);
+        const auto get = [&](const void* src, size_t nb) {
+            Check(hipMemcpyAsync(q, src, nb, hipMemcpyDeviceToHost, engine->stream), "checkpoint");
+            q += nb;
+        };
+        get(l.spins.ptr, l.n);
+        get(l.rng.ptr, 4 * l.n);
+        get(l.masks.ptr, 24ull * l.n_groups);
+        get(l.rot.ptr, 4ull * l.n_groups);
+        get(l.state.ptr, 4ull * l.state_words);
+        Check(hipStreamSynchronize(engine->stream), "checkpoint");
+    });
+}
+
+int swmi_restore(swmi_engine* engine, uint32_t lattice, const void* buffer, uint64_t bytes)
+{
+    return Guard([&] {
+        Lattice& l = Eng(engine)->At(lattice);
+        if (!buffer || bytes < sizeof(CheckpointHeader))
+            throw SwmiError(SWMI_ERR_INVALID, "checkpoint buffer too small");
+        CheckpointHeader h{};
+        std::memcpy(&h, buffer, sizeof(h));
+        if (std::memcmp(h.magic, "SWMICKP1", 8) != 0)
+            throw SwmiError(SWMI_ERR_INVALID, "not an swmi checkpoint");
+        if (h.n0 != l.cfg.n0 || h.n1 != l.cfg.n1 || h.state_words != l.state_words || h.n != l.n || h.n_groups != l.n_groups)
+            throw SwmiError(SWMI_ERR_INVALID, "checkpoint of a " + std::to_string(h.n0) + " x " + std::to_string(h.n1) +
+                                                  " lattice (series words " + std::to_string(h.state_words) +
+                                                  ") does not fit this lattice");
+        if (bytes < CheckpointBytes(l))
+            throw SwmiError(SWMI_ERR_INVALID, "checkpoint truncated");
+        swmi_lattice_config c = l.cfg;
+        c.seed = h.seed;
+        c.flip_rule = h.flip_rule;
+        c.temperature = h.temperature;
+        ValidateConfig(c);
+        Check(hipStreamSynchronize(engine->stream), "restore");
+        const uint8_t* q = static_cast<const uint8_t*>(buffer) + sizeof(h);
+        const auto put = [&](void* dst, size_t nb) {
+            Check(hipMemcpyAsync(dst, q, nb, hipMemcpyHostToDevice, engine->stream), "restore");
+            q += nb;
+        };
+        put(l.spins.ptr, l.n);
+        put(l.rng.ptr, 4 * l.n);
+        put(l.masks.ptr, 24ull * l.n_groups);
+        put(l.rot.ptr, 4ull * l.n_groups);
+        put(l.state.ptr, 4ull * l.state_words);
+        Check(hipStreamSynchronize(engine->stream), "restore");
+        l.cfg.seed = c.seed;
+        l.cfg.flip_rule = c.flip_rule;
+        l.cfg.temperature = c.temperature;
+        l.p_add = BondProbability(c.temperature);
+        l.threshold = BondThreshold(l.p_add);
+        engine->Rebuild();
+        Check(hipStreamSynchronize(engine->stream), "restore");
+    });
+}
+
+int swmi_error_bars(swmi_engine* engine, uint32_t lattice, uint32_t n_bins, double* out)
+{
+    return Guard([&] {
+        if (!out || n_bins < 2)
+            throw SwmiError(SWMI_ERR_INVALID, "null output or fewer than 2 bins");
+        Lattice& l = Eng(engine)->At(lattice);
+        LatState st{};
+        Check(hipMemcpyAsync(&st, l.state.ptr, sizeof(st), hipMemcpyDeviceToHost, engine->stream), "error bars");
+        Check(hipStreamSynchronize(engine->stream), "error bars");
+        const uint32_t stored = std::min(st.n_samples, st.series_cap);
+        const uint32_t per = stored / n_bins;
+        if (per == 0)
+            throw SwmiError(SWMI_ERR_STATE, "not enough samples in the series for " + std::to_string(n_bins) + " bins");
+        const uint32_t used = per * n_bins;
+        std::vector<int64_t> ser(2ull * used);
+        Check(hipMemcpyAsync(ser.data(), l.state.ptr + kLatStateWords, ser.size() * 8, hipMemcpyDeviceToHost, engine->stream),
+              "error bars");
+        Check(hipStreamSynchronize(engine->stream), "error bars");
+        const double n = static_cast<double>(l.n), t = l.cfg.temperature;
+        std::vector<double> be(n_bins, 0.0), be2(n_bins, 0.0), bm(n_bins, 0.0);
+        double s1 = 0, s2 = 0, sm = 0;
+        for (uint32_t i = 0; i < used; ++i)
+        {
+            const double e = -static_cast<double>(ser[2 * i]) / n, m = std::fabs(static_cast<double>(ser[2 * i + 1]) / n);
+            be[i / per] += e;
+            be2[i / per] += e * e;
+            bm[i / per] += m;
+            s1 += e;
+            s2 += e * e;
+            sm += m;
+        }
+        const double mean_e = s1 / used, mean_m = sm / used;
+        double var_be = 0, var_bm = 0;
+        for (uint32_t b = 0; b < n_bins; ++b)
+        {
+            var_be += (be[b] / per - mean_e) * (be[b] / per - mean_e);
+            var_bm += (bm[b] / per - mean_m) * (bm[b] / per - mean_m);
+        }
+        var_be /= (n_bins - 1);
+        var_bm /= (n_bins - 1);
+        const double var_e = s2 / used - mean_e * mean_e;
+        const double c_all = var_e * n / (t * t);
+        // jackknife over bins for C/N
+        double cj_mean = 0;
+        std::vector<double> cj(n_bins);
+        for (uint32_t b = 0; b < n_bins; ++b)
+        {
+            const double m1 = (s1 - be[b]) / (used - per), m2 = (s2 - be2[b]) / (used - per);
+            cj[b] = (m2 - m1 * m1) * n / (t * t);
+            cj_mean += cj[b] / n_bins;
+        }
+        double cj_var = 0;
+        for (uint32_t b = 0; b < n_bins; ++b)
+            cj_var += (cj[b] - cj_mean) * (cj[b] - cj_mean);
+        out[0] = mean_e;
+        out[1] = std::sqrt(var_be / n_bins);
+        out[2] = mean_m;
+        out[3] = std::sqrt(var_bm / n_bins);
+        out[4] = c_all;
+        out[5] = std::sqrt(cj_var * (n_bins - 1) / n_bins);
+        out[6] = var_e > 0 ? 0.5 * var_be * per / var_e : 0.0;
+        out[7] = used;
     });
 }
 

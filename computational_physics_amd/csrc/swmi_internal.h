@@ -66,7 +66,8 @@ namespace swmi
         uint32_t threshold;      // bond iff draw u < threshold
         uint32_t block_base;     // first block of this lattice in the engine work list
         uint32_t n_blocks;       // tiles_x * blocks_y
-        uint32_t flip_rule;
+        uint32_t flip_rule;      // SWMI_FLIP_PARITY (0) or SWMI_FLIP_RANDOM (1)
+        uint32_t seed;           // the lattice's lcg32 seed (keys the random flip rule)
     };
 
     // Per-lattice device counters and FP64 moments (uint32 words; moments start at word kStMoments).
@@ -86,13 +87,14 @@ namespace swmi
         uint32_t it_meas;        // iteration at the end of the warmup
         uint32_t sep2;           // 2 * n_sep
         uint32_t last_sample;    // iteration of the last sample (0: none)
-        uint32_t pad[3];
+        uint32_t flip_key;       // random flip rule: key of the current sweep (written by k_bond_label)
+        uint32_t pad[2];
         double moments[6];       // samples, sum e, sum e^2, sum |m|, sum m^2, sum m^4
         // followed by int64 series[2 * series_cap]
     };
     constexpr uint32_t kLatStateWords = sizeof(LatState) / 4;
     constexpr uint32_t kStIteration = 0, kStSamples = 1, kStErr = 3, kStItEnd = 4, kStActive = 5, kStItMeas = 6,
-                       kStSep2 = 7, kStLastSample = 8, kStMoments = 12;
+                       kStSep2 = 7, kStLastSample = 8, kStFlipKey = 9, kStMoments = 12;
     static_assert(offsetof(LatState, moments) == 4 * kStMoments, "LatState layout");
 
     enum SampleMode : uint32_t

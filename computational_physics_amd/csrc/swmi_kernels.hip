@@ -71,6 +71,30 @@ namespace swmi
         return ReadLane(st, it & 63u) + ((it & 1u) ? 0u : 1u);
     }
 
+    // ---- random flip rule: MurmurHash3 (Austin Appleby, public domain). The cluster with canonical root r
+    // flips iff the top bit of Fmix32(r ^ key) is set; key = MurmurHash3_x86_32 of the 4-byte lcg32
+    // iteration count before the sweep, seeded with the lattice seed. ----
+    __device__ __forceinline__ uint32_t Fmix32(uint32_t h)
+    {
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        return h;
+    }
+
+    __device__ __forceinline__ uint32_t FlipKey(const uint32_t seed, const uint32_t iteration)
+    {
+        uint32_t k = iteration * 0xCC9E2D51u;
+        k = (k << 15) | (k >> 17);
+        k *= 0x1B873593u;
+        uint32_t h = seed ^ k;
+        h = (h << 13) | (h >> 19);
+        h = h * 5u + 0xE6546B64u;
+        return Fmix32(h ^ 4u);
+    }
+
     // ---- union-find in LDS (parents satisfy P[x] <= x; the workgroup shares one array) ----
     __device__ __forceinline__ uint32_t LdsLoad(const uint32_t* p)
     {
@@ -312,7 +336,11 @@ namespace swmi
         const uint32_t it0 = injected ? 0u : stp[kStIteration];
         const bool act = injected || it0 < stp[kStItEnd];
         if (bl.lb == 0 && threadIdx.x == 0)
+        {
             stp[kStActive] = act ? 1u : 0u;
+            if (d.flip_rule != 0u)
+                stp[kStFlipKey] = FlipKey(d.seed, it0);
+        }
         if (!act)
             return;
         const uint32_t it1 = it0 + 1u;
@@ -719,6 +747,8 @@ namespace swmi
         const gptr<uint8_t> spins = Glob(d.spins);
         const uint32_t qmax = bl.nwr * 64u;
         const bool write_labels = (flags & kWriteLabels) != 0u;
+        const bool random_rule = d.flip_rule != 0u;
+        const uint32_t key = random_rule ? Glob(d.state)[kStFlipKey] : 0u;
         const uint32_t sub = threadIdx.x >> 4;
         const uint32_t col = (threadIdx.x & 15u) * 4u;
         bool ok = true;
@@ -802,7 +832,8 @@ namespace swmi
                         }
                         const uint32_t g = R[edge ? q : 0u];
                         root[e] = edge ? g : SiteOf(bl.base, bl.stride, q);
-                        flip |= (root[e] & 1u) << (8u * e);
+                        const uint32_t bit = random_rule ? (Fmix32(root[e] ^ key) >> 31) : (root[e] & 1u);
+                        flip |= bit << (8u * e);
                     }
                     if (do_flip)
                         *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;

@@ -102,7 +102,7 @@ struct swmi_server
         c.seed = j.seed;
         c.spin_seed = j.spin_seed;
         c.tile_rows = 0;
-        c.flip_rule = SWMI_FLIP_PARITY;
+        c.flip_rule = j.flip_rule;
         c.max_samples = 0;
         return c;
     }

@@ -34,6 +34,7 @@ namespace
         std::string measure = "device";
         std::uint32_t seed = 0;
         std::uint32_t tile_rows = 0;
+        std::string flip_rule = "parity";
     };
 
     void Help(const char* prog)
@@ -51,6 +52,7 @@ namespace
                   << "  --temperature      Set temperature (default: 2.2691853)\n"
                   << "  --seed             Set the lcg32 seed (default: 0)\n"
                   << "  --tile_rows        Tile height hint in 64-site rows (default: engine choice)\n"
+                  << "  --flip_rule=<rule> parity (reference: flip iff the root index is odd) or random (default: parity)\n"
                   << "  --measure=<where>  Accumulate samples on the device or the host (default: device)\n";
     }
 
@@ -90,6 +92,8 @@ namespace
                 o.seed = static_cast<std::uint32_t>(std::strtoul(value.c_str(), nullptr, 10));
             else if (key == "--tile_rows")
                 o.tile_rows = static_cast<std::uint32_t>(std::atoi(value.c_str()));
+            else if (key == "--flip_rule")
+                o.flip_rule = value;
         }
         return o;
     }
@@ -102,10 +106,12 @@ namespace
             throw std::runtime_error("Unknown algorithm.");
         if (o.measure != "device" && o.measure != "host")
             throw std::runtime_error("Unknown measurement mode.");
+        if (o.flip_rule != "parity" && o.flip_rule != "random")
+            throw std::runtime_error("Unknown flip rule.");
         auto device = GetDevice(o.target);
         auto& gpu = static_cast<MI355X&>(*device);
 
-        Lattice<2> lattice({o.n0, o.n1}, o.seed, o.tile_rows);
+        Lattice<2> lattice({o.n0, o.n1}, o.seed, o.tile_rows, o.flip_rule == "random" ? SWMI_FLIP_RANDOM : SWMI_FLIP_PARITY);
         std::cout << "Target name: MI355X (" << gpu.Info().arch << ", " << gpu.Concurrency() << " CUs)" << std::endl;
         std::cout << "WARMUP: " << o.n_warmup << " sweeps" << std::endl;
         std::cout << "MEASUREMENT: " << o.n_sweeps << " sweeps" << std::endl;

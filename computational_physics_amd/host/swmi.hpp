@@ -110,6 +110,7 @@ namespace cp
         float engine_temperature = 0.0f;
         std::uint32_t seed = 0;
         std::uint32_t tile_rows = 0;
+        std::uint32_t flip_rule = SWMI_FLIP_PARITY;
 
         template <template <DeviceName> typename, DeviceName>
         friend class SwendsenWang_2D;
@@ -126,7 +127,7 @@ namespace cp
                 c.seed = seed;
                 c.spin_seed = 1;
                 c.tile_rows = tile_rows;
-                c.flip_rule = SWMI_FLIP_PARITY;
+                c.flip_rule = flip_rule;
                 c.max_samples = 0;
                 SwmiCheck(swmi_create(&c, 1, &engine));
                 SwmiCheck(swmi_upload_spins(engine, 0, spins.data()));
@@ -141,11 +142,12 @@ namespace cp
         }
 
     public:
-        // seed: the engine's lcg32 seed; tile_rows: tile height hint (results do not depend on it).
+        // seed: the engine's lcg32 seed; tile_rows: tile height hint (results do not depend on it);
+        // rule: SWMI_FLIP_PARITY (the reference's) or SWMI_FLIP_RANDOM.
         explicit Lattice(const std::array<std::uint32_t, 2>& ext, const std::uint32_t rng_seed = 0,
-                         const std::uint32_t tile_rows_hint = 0)
+                         const std::uint32_t tile_rows_hint = 0, const std::uint32_t rule = SWMI_FLIP_PARITY)
             : extent(ext), num_sites(static_cast<std::size_t>(ext[0]) * ext[1]), spins(num_sites), seed(rng_seed),
-              tile_rows(tile_rows_hint)
+              tile_rows(tile_rows_hint), flip_rule(rule)
         {
             // lattice.cpp:24-26: srand48(1); spin = drand48() < 0.5 ? 0 : 1
             unsigned short xs[3] = {0x330E, 1, 0};

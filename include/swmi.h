@@ -39,7 +39,10 @@ enum
 
 enum
 {
-    SWMI_FLIP_PARITY = 0 /* flip the cluster iff its canonical root index is odd (reference rule) */
+    SWMI_FLIP_PARITY = 0, /* flip the cluster iff its canonical root index is odd (reference rule) */
+    SWMI_FLIP_RANDOM = 1  /* flip the cluster with root r iff the top bit of fmix32(r ^ key) is set, key =
+                           * MurmurHash3_x86_32(lcg32 iteration before the sweep, 4 bytes LE; seed) --
+                           * one random bit per cluster, site 0 not frozen (no counterpart in the reference) */
 };
 
 typedef struct swmi_engine swmi_engine;
@@ -52,7 +55,7 @@ typedef struct
     uint32_t seed;        /* lcg32 group seeds: seed*0x9E3779B1 + g + 1 (seed 0: g + 1) */
     uint32_t spin_seed;   /* hot start: srand48(spin_seed); spin = drand48() < 0.5 ? 0 : 1 (1: reference) */
     uint32_t tile_rows;   /* tile height in 64-site wave rows (0: default); results do not depend on it */
-    uint32_t flip_rule;   /* SWMI_FLIP_PARITY */
+    uint32_t flip_rule;   /* SWMI_FLIP_PARITY or SWMI_FLIP_RANDOM */
     uint32_t max_samples; /* capacity of the on-device (E, M) time series (0: no series, moments only) */
 } swmi_lattice_config;
 
@@ -149,6 +152,24 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
 int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uint64_t* masks, uint32_t* rot,
                     uint32_t* iteration);
 
+/* ---- checkpoint / resume of a lattice's Markov chain: spins (with the lcg32 selector bits), per-site
+ * lcg32 states, selector planes, rotations, counters, FP64 moments and the (E, M) series. Labels are
+ * per-sweep scratch and are not saved. A restored lattice continues bit for bit as the saved one would
+ * have. The reference has no checkpoint (its state lives in a function-local static,
+ * ising/include/lattice.hpp:56). Format: a 64-byte header ("SWMICKP1", extents, seed, flip rule,
+ * temperature, word counts), then the arrays in that order, little endian. ---- */
+int swmi_checkpoint_size(swmi_engine* engine, uint32_t lattice, uint64_t* bytes);
+int swmi_checkpoint(swmi_engine* engine, uint32_t lattice, void* buffer, uint64_t bytes);
+/* The target lattice must have the same extents and series capacity; its temperature, seed and flip rule
+ * become the checkpoint's. */
+int swmi_restore(swmi_engine* engine, uint32_t lattice, const void* buffer, uint64_t bytes);
+
+/* ---- error bars from the on-device (E, M) series (max_samples > 0): out[0..7] = {<e>, err <e> (binning
+ * into n_bins bins), <|m|>, err <|m|>, C/N = N (<e^2> - <e>^2) / T^2, jackknife err C/N over the bins,
+ * tau_int of e (samples, from the binned variance: 0.5 * var_bin_mean * n_per_bin / var_sample), samples
+ * used}, e = -E/N, m = M/N. The reference prints the means only (ising/src/ising_2d.cpp:152-162). ---- */
+int swmi_error_bars(swmi_engine* engine, uint32_t lattice, uint32_t n_bins, double* out);
+
 /* Invariant violations seen by the kernels (bit 0: a label chase left [0, n)); 0 in a correct run. */
 int swmi_get_error_flags(swmi_engine* engine, uint32_t lattice, uint32_t* flags);
 
@@ -185,6 +206,7 @@ typedef struct
     uint32_t spin_seed;      /* hot start seed */
     uint32_t warmup;         /* unsampled sweeps first, a multiple of n_sep */
     uint32_t sweeps;         /* measured sweeps after the warmup (> 0) */
+    uint32_t flip_rule;      /* SWMI_FLIP_PARITY or SWMI_FLIP_RANDOM */
     uint64_t tag;            /* caller's value, returned with the result */
 } swmi_job;
 

@@ -20,6 +20,8 @@
  *   or_observables          brute-force numpy sums
  *   or_ps_*                 composition of the pinned pieces above, plus the rotation invariant
  *                           checked against a full or_lcg32 per group
+ *   or_murmur3_32/or_fmix32 MurmurHash3_x86_32 published test vectors (SMHasher); or_flip_random is
+ *                           their composition with the pinned labeller
  */
 #include "oracle.h"
 
@@ -394,6 +396,77 @@ void or_flip(size_t n, uint8_t* spins, const uint32_t* labels)
 }
 
 /* ------------------------------------------------------------------------------------------------
+ * The engine's random flip rule (no counterpart in the reference, whose rule is the root parity,
+ * swendsen_wang_cpu.cpp:183-205): the cluster with canonical root r flips iff the top bit of
+ * fmix32(r ^ key) is set, key = MurmurHash3_x86_32 of the 4-byte little-endian lcg32 iteration count
+ * before the sweep, seeded with the lattice's lcg32 seed. fmix32 and MurmurHash3_x86_32 are Austin
+ * Appleby's public-domain MurmurHash3 (SMHasher); they are pinned by its published test vectors.
+ * ------------------------------------------------------------------------------------------------ */
+uint32_t or_fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6BU;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35U;
+    h ^= h >> 16;
+    return h;
+}
+
+static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+uint32_t or_murmur3_32(const uint8_t* data, size_t len, uint32_t seed)
+{
+    const uint32_t c1 = 0xCC9E2D51U, c2 = 0x1B873593U;
+    uint32_t h = seed;
+    const size_t nblocks = len / 4;
+    for (size_t i = 0; i < nblocks; ++i)
+    {
+        const uint8_t* b = data + 4 * i;
+        uint32_t k = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+        k *= c1;
+        k = rotl32(k, 15);
+        k *= c2;
+        h ^= k;
+        h = rotl32(h, 13);
+        h = h * 5U + 0xE6546B64U;
+    }
+    const uint8_t* tail = data + 4 * nblocks;
+    uint32_t k1 = 0;
+    switch (len & 3U)
+    {
+    case 3:
+        k1 ^= (uint32_t)tail[2] << 16;
+        /* fall through */
+    case 2:
+        k1 ^= (uint32_t)tail[1] << 8;
+        /* fall through */
+    case 1:
+        k1 ^= tail[0];
+        k1 *= c1;
+        k1 = rotl32(k1, 15);
+        k1 *= c2;
+        h ^= k1;
+        break;
+    default:
+        break;
+    }
+    h ^= (uint32_t)len;
+    return or_fmix32(h);
+}
+
+uint32_t or_flip_key(uint32_t seed, uint32_t iteration)
+{
+    const uint8_t b[4] = {(uint8_t)iteration, (uint8_t)(iteration >> 8), (uint8_t)(iteration >> 16), (uint8_t)(iteration >> 24)};
+    return or_murmur3_32(b, 4, seed);
+}
+
+void or_flip_random(size_t n, uint8_t* spins, const uint32_t* labels, uint32_t key)
+{
+    for (size_t i = 0; i < n; ++i)
+        spins[i] ^= (uint8_t)(or_fmix32(labels[i] ^ key) >> 31);
+}
+
+/* ------------------------------------------------------------------------------------------------
  * The reference CPU path SwendsenWang_2D<LCG32, CPU>.
  *   Initialize   swendsen_wang.cpp:41-91: NUM_THREADS generators, generator t seeded with t+1
  *   Update       swendsen_wang.cpp:94-112
@@ -585,6 +658,14 @@ void or_ps_bonds(uint32_t n0, uint32_t n1, const uint8_t* spins, const uint32_t*
 void or_ps_sweep(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spins, uint32_t* states,
                  const uint64_t* masks, uint32_t* rot, uint32_t* iteration, uint8_t* bonds_out, uint32_t* labels_out)
 {
+    or_ps_sweep_rule(n0, n1, threshold, spins, states, masks, rot, iteration, bonds_out, labels_out, 0U, 0U);
+}
+
+void or_ps_sweep_rule(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spins, uint32_t* states,
+                      const uint64_t* masks, uint32_t* rot, uint32_t* iteration, uint8_t* bonds_out, uint32_t* labels_out,
+                      uint32_t flip_rule, uint32_t seed)
+{
+    const uint32_t key = or_flip_key(seed, *iteration);
     const size_t n = (size_t)n0 * n1;
     const uint32_t n_groups = (uint32_t)((n + 63) / 64);
     const size_t npad = (size_t)n_groups * 64;
@@ -597,7 +678,10 @@ void or_ps_sweep(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spins, u
     *iteration += 2U;
     or_ps_bonds(n0, n1, spins, u1, states, threshold, bonds);
     or_label(n0, n1, 16, 8, bonds, labels);
-    or_flip(n, spins, labels);
+    if (flip_rule == 1U)
+        or_flip_random(n, spins, labels, key);
+    else
+        or_flip(n, spins, labels);
     if (bonds_out)
         memcpy(bonds_out, bonds, n);
     if (labels_out)

@@ -61,6 +61,11 @@ uint64_t or_threshold_mismatches(float p_add, uint64_t threshold);
  * Bond bytes: bit 0 = bond to (x+1 mod N0, y), bit 1 = bond to (x, y+1 mod N1). Site index y*N0+x. */
 void or_label(uint32_t n0, uint32_t n1, uint32_t tile_x, uint32_t tile_y, const uint8_t* bonds, uint32_t* labels);
 void or_flip(size_t n, uint8_t* spins, const uint32_t* labels);
+/* engine random flip rule: MurmurHash3 finaliser of (root ^ key), key = MurmurHash3_x86_32(iteration, seed) */
+uint32_t or_fmix32(uint32_t h);
+uint32_t or_murmur3_32(const uint8_t* data, size_t len, uint32_t seed);
+uint32_t or_flip_key(uint32_t seed, uint32_t iteration);
+void or_flip_random(size_t n, uint8_t* spins, const uint32_t* labels, uint32_t key);
 void or_observables(uint32_t n0, uint32_t n1, const uint8_t* spins, int64_t* energy, int64_t* magnetization);
 
 /* ---- the reference CPU path, SwendsenWang_2D<LCG32, CPU> (ising/src/swendsen_wang.cpp:41-112,
@@ -87,6 +92,9 @@ void or_ps_bonds(uint32_t n0, uint32_t n1, const uint8_t* spins, const uint32_t*
  * flip_rule 0: spin ^= root & 1 (reference). bonds_out / labels_out may be NULL. */
 void or_ps_sweep(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spins, uint32_t* states,
                  const uint64_t* masks, uint32_t* rot, uint32_t* iteration, uint8_t* bonds_out, uint32_t* labels_out);
+void or_ps_sweep_rule(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spins, uint32_t* states,
+                      const uint64_t* masks, uint32_t* rot, uint32_t* iteration, uint8_t* bonds_out, uint32_t* labels_out,
+                      uint32_t flip_rule, uint32_t seed);
 
 #ifdef __cplusplus
 }

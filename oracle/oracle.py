@@ -56,6 +56,13 @@ def lib() -> ctypes.CDLL:
     L.or_threshold_mismatches.restype = u64
     L.or_label.argtypes = [u32, u32, u32, u32, vp, vp]
     L.or_flip.argtypes = [ctypes.c_size_t, vp, vp]
+    L.or_fmix32.argtypes = [u32]
+    L.or_fmix32.restype = u32
+    L.or_murmur3_32.argtypes = [ctypes.c_char_p, ctypes.c_size_t, u32]
+    L.or_murmur3_32.restype = u32
+    L.or_flip_key.argtypes = [u32, u32]
+    L.or_flip_key.restype = u32
+    L.or_flip_random.argtypes = [ctypes.c_size_t, vp, vp, u32]
     L.or_observables.argtypes = [u32, u32, vp, P(ctypes.c_int64), P(ctypes.c_int64)]
     L.or_refcpu_create.argtypes = [u32] * 6
     L.or_refcpu_create.restype = vp
@@ -67,6 +74,7 @@ def lib() -> ctypes.CDLL:
     L.or_ps_rng_update.argtypes = [u32, vp, vp, vp, u32]
     L.or_ps_bonds.argtypes = [u32, u32, vp, vp, vp, u64, vp]
     L.or_ps_sweep.argtypes = [u32, u32, u64, vp, vp, vp, vp, P(u32), vp, vp]
+    L.or_ps_sweep_rule.argtypes = [u32, u32, u64, vp, vp, vp, vp, P(u32), vp, vp, u32, u32]
     _lib = L
     return L
 
@@ -143,8 +151,9 @@ class RefCpu:
 class PerSite:
     """The MI355X engine's per-site lcg32 layout and sweep, restated on the CPU."""
 
-    def __init__(self, n0, n1, temperature, seed=0, spin_seed=1):
+    def __init__(self, n0, n1, temperature, seed=0, spin_seed=1, flip_rule=0):
         self.n0, self.n1 = n0, n1
+        self.seed, self.flip_rule = seed, flip_rule
         self.n = n0 * n1
         self.groups = (self.n + 63) // 64
         self.states = np.empty(self.groups * 64, dtype=np.uint32)
@@ -162,7 +171,20 @@ class PerSite:
     def sweep(self, record=False):
         bonds = np.empty(self.n, dtype=np.uint8) if record else None
         labels = np.empty(self.n, dtype=np.uint32) if record else None
-        lib().or_ps_sweep(self.n0, self.n1, self.threshold, self.spins.ctypes.data, self.states.ctypes.data,
-                          self.masks.ctypes.data, self.rot.ctypes.data, ctypes.byref(self.iteration),
-                          None if bonds is None else bonds.ctypes.data, None if labels is None else labels.ctypes.data)
+        lib().or_ps_sweep_rule(self.n0, self.n1, self.threshold, self.spins.ctypes.data, self.states.ctypes.data,
+                               self.masks.ctypes.data, self.rot.ctypes.data, ctypes.byref(self.iteration),
+                               None if bonds is None else bonds.ctypes.data, None if labels is None else labels.ctypes.data,
+                               self.flip_rule, self.seed & 0xFFFFFFFF)
         return bonds, labels
+
+
+def murmur3_32(data: bytes, seed: int) -> int:
+    return int(lib().or_murmur3_32(data, len(data), seed & 0xFFFFFFFF))
+
+
+def fmix32(h: int) -> int:
+    return int(lib().or_fmix32(h & 0xFFFFFFFF))
+
+
+def flip_key(seed: int, iteration: int) -> int:
+    return int(lib().or_flip_key(seed & 0xFFFFFFFF, iteration & 0xFFFFFFFF))

@@ -162,3 +162,29 @@ def test_large_lattice_one_sweep_bit_exact(swmi, oracle):
         np.testing.assert_array_equal(e.labels(), rl)
         np.testing.assert_array_equal(e.spins(), ref.spins)
         assert e.error_flags() == 0
+
+
+@pytest.mark.parametrize("n0,n1,T,tile_rows", [(64, 64, TC, 0), (128, 96, 2.0, 4), (32, 32, 2.6, 0), (256, 128, TC, 16),
+                                               (16, 64, 1.5, 2), (512, 64, 3.0, 64)])
+def test_random_flip_rule_bit_exact(swmi, oracle, n0, n1, T, tile_rows):
+    """SWMI_FLIP_RANDOM: spins of every sweep equal the oracle's restatement; site 0 is not frozen."""
+    print(f"random flips {n0}x{n1} T {T}", flush=True)
+    ref = oracle.PerSite(n0, n1, T, seed=21, spin_seed=3, flip_rule=1)
+    s0 = []
+    with swmi.Engine(swmi.LatticeConfig(n0, n1, T, seed=21, spin_seed=3, tile_rows=tile_rows, flip_rule=1)) as e:
+        for k in range(1, 21):
+            e.sweep(1)
+            ref.sweep()
+            s = e.spins()
+            np.testing.assert_array_equal(s, ref.spins, err_msg=f"sweep {k}")
+            s0.append(int(s[0]))
+        assert e.error_flags() == 0
+    assert len(set(s0)) == 2, "site 0 never changed under the random rule"
+
+
+def test_parity_rule_freezes_site_zero_on_the_gpu(swmi):
+    with _engine(swmi, 128, 128, seed=4) as e:
+        first = int(e.spins()[0])
+        for _ in range(10):
+            e.sweep(3)
+            assert int(e.spins()[0]) == first

@@ -232,3 +232,47 @@ def test_exact_values_match_reference_tool():
                              check=True, capture_output=True, text=True).stdout.split()
         assert abs(float(out[1]) - v["U_per_spin"]) < 1e-9
         assert abs(float(out[2]) - v["C_per_spin"]) < 1e-6
+
+
+# MurmurHash3_x86_32 test vectors published with SMHasher (Austin Appleby) and widely reproduced.
+MURMUR3_KATS = [(b"", 0, 0x00000000), (b"", 1, 0x514E28B7), (b"", 0xFFFFFFFF, 0x81F16F39),
+                (b"\x00\x00\x00\x00", 0, 0x2362F9DE), (b"\xff\xff\xff\xff", 0, 0x76293B50),
+                (b"\x21\x43\x65\x87", 0, 0xF55B516B), (b"a", 0x9747B28C, 0x7FA09EA6), (b"aa", 0x9747B28C, 0x5D211726),
+                (b"aaa", 0x9747B28C, 0x283E0130), (b"aaaa", 0x9747B28C, 0x5A97808A), (b"abcd", 0x9747B28C, 0xF0478627),
+                (b"Hello, world!", 0x9747B28C, 0x24884CBA),
+                (b"The quick brown fox jumps over the lazy dog", 0x9747B28C, 0x2FA826CD)]
+
+
+@pytest.mark.parametrize("data,seed,want", MURMUR3_KATS)
+def test_murmur3_matches_published_vectors(oracle, data, seed, want):
+    assert oracle.murmur3_32(data, seed) == want
+
+
+def test_flip_key_is_murmur3_of_the_iteration(oracle):
+    for seed, it in [(0, 0), (7, 2), (0xDEADBEEF, 123456), (1, 0xFFFFFFFE)]:
+        assert oracle.flip_key(seed, it) == oracle.murmur3_32(int(it).to_bytes(4, "little"), seed)
+
+
+def test_random_flip_flips_whole_clusters_half_of_the_time(oracle):
+    """Every site of a cluster flips together, the decision depends on the root only, and over many roots
+    about half of the clusters flip."""
+    import ctypes
+    rng = np.random.default_rng(5)
+    n0, n1 = 64, 64
+    bonds = ((rng.random(n0 * n1) < 0.4).astype(np.uint8) | ((rng.random(n0 * n1) < 0.4).astype(np.uint8) << 1))
+    labels = oracle.label(n0, n1, bonds)
+    spins = rng.integers(0, 2, n0 * n1).astype(np.uint8)
+    key = oracle.flip_key(3, 10)
+    out = spins.copy()
+    oracle.lib().or_flip_random(out.size, out.ctypes.data, labels.ctypes.data, key)
+    flipped = out != spins
+    roots = np.unique(labels)
+    for r in roots:
+        members = labels == r
+        assert flipped[members].all() or not flipped[members].any()
+        assert bool(flipped[members][0]) == bool(oracle.fmix32(int(r) ^ key) >> 31)
+    frac = np.mean([flipped[labels == r][0] for r in roots])
+    assert 0.4 < frac < 0.6, frac
+    # site 0 is not frozen under this rule: over keys its cluster flips about half of the time
+    hits = [oracle.fmix32(int(labels[0]) ^ oracle.flip_key(3, 2 * k)) >> 31 for k in range(400)]
+    assert 0.4 < np.mean(hits) < 0.6
