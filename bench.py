@@ -101,6 +101,11 @@ def batched_leg(cpa, warmup: int = 1000, sweeps: int = 5000, alone_sweeps: int =
         tuned[L] = cpa.autotune(cpa.LatticeConfig(L, L, 2.2691853, seed=1), sweeps=10)[0]
     cfgs = [cpa.LatticeConfig(L, L, t, seed=1000 + i, spin_seed=1 + i) for i, (L, t) in enumerate(zip(ls, ts))]
     sites = sum(L * L for L in ls)
+    # the batch is tuned as a batch (one tile height for all lattices); the one-by-one baseline below uses
+    # each lattice's own best height (tuned alone above)
+    batch_rows, batch_ms = cpa.autotune_batch(cfgs, sweeps=20)
+    for c in cfgs:
+        c.tile_rows = batch_rows
     with cpa.Engine(cfgs) as eng:
         eng.sweep(warmup)
         eng.synchronize()
@@ -113,6 +118,7 @@ def batched_leg(cpa, warmup: int = 1000, sweeps: int = 5000, alone_sweeps: int =
     # the same lattices one by one through the plain path (one engine each; timed sample of the sweeps)
     t_alone = 0.0
     for c in cfgs:
+        c.tile_rows = 0   # the height tuned for this extent alone
         with cpa.Engine(c) as e1:
             e1.sweep(10)
             e1.synchronize()
@@ -123,8 +129,39 @@ def batched_leg(cpa, warmup: int = 1000, sweeps: int = 5000, alone_sweeps: int =
     alone = alone_sweeps * sites / t_alone
     return {"value": batched, "unit": "spin-updates/s", "lattices": len(cfgs), "sites": sites, "warmup": warmup,
             "sweeps": sweeps, "seconds": dt, "one_by_one": alone, "one_by_one_sample_sweeps": alone_sweeps,
-            "speedup_vs_one_by_one": batched / alone, "tile_rows_by_L": tuned, "error_flags": flags,
+            "speedup_vs_one_by_one": batched / alone, "tile_rows_alone_by_L": tuned, "batch_tile_rows": batch_rows,
+            "batch_autotune_ms_per_sweep": dict(zip(["1", "2", "4", "8", "16", "32", "64"], batch_ms)), "error_flags": flags,
             "workload": "256 lattices, L=2^k k~U{5..10}, T~U[1.8,2.8], default_rng(0); one engine, graph-captured"}
+
+
+def plain_configs_leg(cpa, warmup: int = 50, sweeps: int = 300) -> dict:
+    """The other plain-path configurations: 1024^2 at T_c with the default and the autotuned tile height,
+    4096^2 and 8192^2 at T in {2.0, 2.2691853, 2.6}; sweep-only rates and the 4096^2 rate of the reference
+    protocol (observables every 10 sweeps inside the timed region)."""
+    def rate(cfg, n_sweeps, sep=0):
+        with cpa.Engine(cfg) as e:
+            e.sweep(warmup)
+            e.synchronize()
+            t0 = time.perf_counter()
+            if sep:
+                e.run(n_sweeps, sep)
+            else:
+                e.sweep(n_sweeps)
+            e.synchronize()
+            dt = time.perf_counter() - t0
+            assert e.error_flags() == 0
+        return n_sweeps * cfg.n0 * cfg.n1 / dt
+    out = {}
+    default = rate(cpa.LatticeConfig(1024, 1024, T_C, seed=0, tile_rows=16), 1000)
+    best, ms = cpa.autotune(cpa.LatticeConfig(1024, 1024, T_C, seed=0), sweeps=20)
+    tuned = rate(cpa.LatticeConfig(1024, 1024, T_C, seed=0, tile_rows=best), 1000)
+    out["1024_Tc"] = {"tile_rows_16": default, "autotuned": tuned, "autotuned_tile_rows": best,
+                      "autotune_ms_per_sweep": dict(zip(["1", "2", "4", "8", "16", "32", "64"], ms))}
+    for n in (4096, 8192):
+        for t in (2.0, T_C, 2.6):
+            out[f"{n}_T{t}"] = rate(cpa.LatticeConfig(n, n, t, seed=0), sweeps if n == 4096 else sweeps // 2)
+    out["4096_Tc_with_observables_every_10"] = rate(cpa.LatticeConfig(4096, 4096, T_C, seed=0, max_samples=0), 1000, sep=10)
+    return out
 
 
 def served_leg(cpa, capacity: float, n_jobs: int = 32, load: float = 0.8, seed: int = 1) -> dict:
@@ -173,6 +210,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--served-jobs", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run a device copy of BYTES bytes (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -235,6 +273,9 @@ def main() -> int:
         "error_flags": flags,
     }
     log(f"plain {n}x{n}: {value:.3e} spin-updates/s, {ms_per_step * 1e3:.1f} us/sweep")
+    if not args.no_configs:
+        result["plain_configs"] = plain_configs_leg(cpa)
+        log("plain configs done")
     if not args.no_batched:
         result["batched"] = batched_leg(cpa)
         log(f"batched: {result['batched']['value']:.3e} spin-updates/s, "

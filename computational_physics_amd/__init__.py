@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 
 __all__ = ["SwmiError", "LatticeConfig", "Engine", "Job", "Server", "library_path", "load_library", "device_info",
-           "copy_calibration", "autotune", "REFERENCE_TEMPERATURE"]
+           "copy_calibration", "autotune", "autotune_batch", "REFERENCE_TEMPERATURE"]
 
 FLIP_PARITY = 0   # flip a cluster iff its canonical root index is odd (the reference rule)
 FLIP_RANDOM = 1   # flip a cluster on one random bit of its root (MurmurHash3 of root, sweep, seed)
@@ -126,6 +126,7 @@ def load_library() -> ctypes.CDLL:
         "swmi_error_bars": (ctypes.c_int, [vp, u32, u32, P(f64)]),
         "swmi_get_tile_rows": (ctypes.c_int, [vp, u32, P(u32)]),
         "swmi_autotune": (ctypes.c_int, [P(_Config), u32, P(u32), P(f32)]),
+        "swmi_autotune_batch": (ctypes.c_int, [P(_Config), u32, u32, P(u32), P(f32)]),
         "swmi_set_schedule": (ctypes.c_int, [vp, u32, u32, u32, u32]),
         "swmi_run_scheduled": (ctypes.c_int, [vp, u32, u32]),
         "swmi_get_progress": (ctypes.c_int, [vp, u32, P(u32), P(u32)]),
@@ -172,6 +173,16 @@ def autotune(config: "LatticeConfig", sweeps: int = 20) -> tuple[int, list[float
     best = ctypes.c_uint32()
     ms = (ctypes.c_float * 7)()
     _check(load_library().swmi_autotune(ctypes.byref(config._c()), sweeps, ctypes.byref(best), ms))
+    return best.value, list(ms)
+
+
+def autotune_batch(configs, sweeps: int = 20) -> tuple[int, list[float]]:
+    """Time plain sweeps of the whole batch with one tile height for all lattices (1..64); returns the fastest
+    height and the ms per sweep of each."""
+    arr = (_Config * len(configs))(*[c._c() for c in configs])
+    best = ctypes.c_uint32()
+    ms = (ctypes.c_float * 7)()
+    _check(load_library().swmi_autotune_batch(arr, len(configs), sweeps, ctypes.byref(best), ms))
     return best.value, list(ms)
 
 

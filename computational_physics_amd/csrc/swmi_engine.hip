@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -197,6 +198,9 @@ using namespace swmi;
 
 swmi_engine::swmi_engine()
 {
+    // SWMI_GRAPHS=0 launches every kernel one by one (same results): for tools that trace each launch
+    if (const char* g = std::getenv("SWMI_GRAPHS"))
+        use_graphs = std::strcmp(g, "0") != 0;
     RequireGfx950(0);
     Check(hipSetDevice(0), "hipSetDevice");
     Check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -449,13 +453,16 @@ namespace
         return e;
     }
 
-    // Average device time of a plain sweep of one lattice of this configuration (HIP events).
-    float TimeSweeps(const swmi_lattice_config& c, const uint32_t sweeps)
+    // Average device time of a plain sweep of an engine holding these lattices (HIP events).
+    float TimeSweeps(const swmi_lattice_config* cs, const uint32_t n, const uint32_t sweeps)
     {
         swmi_engine e;
-        e.lattices.push_back(std::make_unique<Lattice>());
-        e.lattices.back()->Allocate(c);
-        e.lattices.back()->Initialize(c, e.stream);
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            e.lattices.push_back(std::make_unique<Lattice>());
+            e.lattices.back()->Allocate(cs[i]);
+            e.lattices.back()->Initialize(cs[i], e.stream);
+        }
         e.Rebuild();
         e.Run(4, 0, -1);
         hipEvent_t a, b;
@@ -1069,7 +1076,7 @@ int swmi_autotune(const swmi_lattice_config* config, uint32_t sweeps, uint32_t* 
             swmi_lattice_config c = *config;
             c.tile_rows = twr;
             c.max_samples = 0;
-            const float ms = TimeSweeps(c, sweeps);
+            const float ms = TimeSweeps(&c, 1, sweeps);
             if (ms_per_sweep)
                 ms_per_sweep[i] = ms;
             if (best == 0 || ms < best_ms)
@@ -1081,6 +1088,39 @@ int swmi_autotune(const swmi_lattice_config* config, uint32_t sweeps, uint32_t* 
         {
             std::lock_guard<std::mutex> lock(g_tune_mutex);
             g_tuned[{config->n0, config->n1}] = best;
+        }
+        if (best_tile_rows)
+            *best_tile_rows = best;
+    });
+}
+
+int swmi_autotune_batch(const swmi_lattice_config* configs, uint32_t n_lattices, uint32_t sweeps, uint32_t* best_tile_rows,
+                        float* ms_per_sweep)
+{
+    return Guard([&] {
+        if (!configs || n_lattices == 0 || sweeps == 0)
+            throw SwmiError(SWMI_ERR_INVALID, "null configs, no lattices or zero sweeps");
+        for (uint32_t i = 0; i < n_lattices; ++i)
+            ValidateConfig(configs[i]);
+        constexpr uint32_t kCand[7] = {1, 2, 4, 8, 16, 32, 64};
+        std::vector<swmi_lattice_config> cs(configs, configs + n_lattices);
+        uint32_t best = 0;
+        float best_ms = 0.0f;
+        for (uint32_t i = 0; i < 7; ++i)
+        {
+            for (auto& c : cs)
+            {
+                c.tile_rows = kCand[i];
+                c.max_samples = 0;
+            }
+            const float ms = TimeSweeps(cs.data(), n_lattices, sweeps);
+            if (ms_per_sweep)
+                ms_per_sweep[i] = ms;
+            if (best == 0 || ms < best_ms)
+            {
+                best = kCand[i];
+                best_ms = ms;
+            }
         }
         if (best_tile_rows)
             *best_tile_rows = best;

@@ -102,7 +102,8 @@ int swmi_get_threshold(const swmi_engine* engine, uint32_t lattice, float* p_add
 int swmi_sweep(swmi_engine* engine, uint32_t n_sweeps);
 /* Launch mode: captured hipGraphs of 16 sweeps (or of sample groups) replayed for long sweep loops
  * (enable = 1, the default) or every kernel launched one by one (0). Results are identical. The
- * reference launches every kernel and synchronises after each (swendsen_wang_amd_gpu.cpp:253-492). */
+ * reference launches every kernel and synchronises after each (swendsen_wang_amd_gpu.cpp:253-492).
+ * New engines start with graphs unless the environment variable SWMI_GRAPHS is "0". */
 int swmi_set_graphs(swmi_engine* engine, int enable);
 /* Tile height (in 64-site wave rows) the lattice runs with. */
 int swmi_get_tile_rows(const swmi_engine* engine, uint32_t lattice, uint32_t* tile_rows);
@@ -112,6 +113,12 @@ int swmi_get_tile_rows(const swmi_engine* engine, uint32_t lattice, uint32_t* ti
  * tile_rows = 0 use it. Spins, labels and random streams do not depend on the tile height. The
  * reference fixes the tile from the wavefront size (ising/src/swendsen_wang.cpp:16-31). */
 int swmi_autotune(const swmi_lattice_config* config, uint32_t sweeps, uint32_t* best_tile_rows, float* ms_per_sweep);
+/* Batch autotuner: times plain sweeps of an engine holding all n_lattices lattices with one tile height
+ * for all of them (each of {1, 2, 4, 8, 16, 32, 64}, clamped per lattice) and returns the height with the
+ * highest aggregate throughput. A batch prefers taller tiles than its small lattices would alone: its
+ * blocks already fill the device, and taller tiles mean fewer block edges to merge. */
+int swmi_autotune_batch(const swmi_lattice_config* configs, uint32_t n_lattices, uint32_t sweeps, uint32_t* best_tile_rows,
+                        float* ms_per_sweep);
 /* Per-lattice schedule, relative to the lattice's current sweep count: it advances `sweeps` more sweeps
  * (0: no limit) and then stays unchanged while the other lattices of the engine go on; swmi_run_scheduled
  * samples it after every n_sep-th sweep past the first `warmup`. The measurement protocol is that of

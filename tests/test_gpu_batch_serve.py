@@ -186,3 +186,16 @@ def test_server_admits_under_a_small_memory_budget(swmi):
         mom, em = _plain_job_result(swmi, jobs[k])
         np.testing.assert_array_equal(out[k]["moments"], mom)
         assert (out[k]["energy"], out[k]["magnetization"]) == em
+
+
+def test_batch_autotune_returns_a_height_and_results_do_not_depend_on_it(swmi):
+    cfgs = [_cfg(swmi, n0, n1, T, seed=30 + i, spin_seed=40 + i) for i, (n0, n1, T, _) in enumerate(MIXED[:6])]
+    best, ms = swmi.autotune_batch(cfgs, sweeps=3)
+    print(f"batch autotune: best {best}, ms {ms}", flush=True)
+    assert best in (1, 2, 4, 8, 16, 32, 64) and all(m > 0 for m in ms)
+    tuned = [_cfg(swmi, c.n0, c.n1, c.temperature, seed=c.seed, spin_seed=c.spin_seed, tile_rows=best) for c in cfgs]
+    with swmi.Engine(tuned) as a, swmi.Engine(cfgs) as b:
+        a.sweep(12)
+        b.sweep(12)
+        for i in range(len(cfgs)):
+            np.testing.assert_array_equal(a.spins(i), b.spins(i), err_msg=f"lattice {i}")
