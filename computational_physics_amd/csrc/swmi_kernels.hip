@@ -156,22 +156,26 @@ namespace swmi
 
     // Parents satisfy L[x] <= x, so a chase ends within n steps; a value outside [0, n) can only come
     // from a broken invariant, which is reported through the lattice's error flag instead of being
-    // dereferenced. A start node more than one hop from its root is pointed at the root.
+    // dereferenced. The first nodes of the path are pointed at the root found (path compression; the
+    // atomics return nothing, so the wave does not wait for them).
     __device__ __forceinline__ bool GlobalFind(gptr<uint32_t> L, uint32_t& x, const uint32_t n)
     {
-        const uint32_t x0 = x;
-        uint32_t p = CachedLoad(&L[x]);
+        uint32_t path[3];
         uint32_t hops = 0;
+        uint32_t p = CachedLoad(&L[x]);
         while (p != x)
         {
             if (p >= n || p > x)
                 return false;
+            if (hops < 3)
+                path[hops] = x;
+            ++hops;
             x = p;
             p = CachedLoad(&L[x]);
-            ++hops;
         }
-        if (hops > 1)
-            (void)__hip_atomic_fetch_min(&L[x0], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // path[k] (k < min(hops, 3)) is a node whose parent is not yet known to be the root x
+        for (uint32_t k = 0; k + 1 < hops && k < 3; ++k)
+            (void)__hip_atomic_fetch_min(&L[path[k]], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return true;
     }
 
@@ -651,8 +655,9 @@ namespace swmi
     // =============================================================================================
     // k_merge: unions across block edges (reference MergeLabels_Kernel, swendsen_wang_amd_gpu.cpp:280-329)
     // =============================================================================================
-    // One workgroup per block: wavefront w takes the right-edge word of tile w (one lane per lattice row),
-    // wavefront 0 also the down edge (one lane per column), so the unions of a block run in parallel.
+    // One two-wave workgroup per block: wavefront 0 takes the right edge (one lane per lattice row),
+    // wavefront 1 the down edge (one lane per column). Two waves per block keep every block of a 4096^2
+    // lattice resident at once, so the latency-bound union chains of all blocks overlap.
     // bit `lane` of `bits`: a bond from the site labelled *pa (this block) to the site labelled *pb (block
     // at base_b); same_line: lanes lane-1 and lane are neighbours along the edge (runs deduplicate)
     __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const gptr<const uint16_t> pa,
@@ -670,14 +675,25 @@ namespace swmi
             a = SiteOf(bl.base, bl.stride, va & 0x7FFFu);
             bq = SiteOf(base_b, bl.stride, vb & 0x7FFFu);
         }
-        const uint32_t ap = static_cast<uint32_t>(__shfl(static_cast<int>(a), static_cast<int>(lane - 1u)));
-        const uint32_t bp = static_cast<uint32_t>(__shfl(static_cast<int>(bq), static_cast<int>(lane - 1u)));
-        const bool dup = same_line && LaneBit(bits << 1) && ap == a && bp == bq;
-        if (bit && !dup && a != bq)
-            GlobalUnion(Glob(d.parent), a, bq, d.n0 * d.n1, err);
+        // one union per distinct pair of roots: the lowest lane holding a pair unions it for all lanes
+        // holding the same pair (below T_c most bonds of an edge join the same two clusters)
+        (void)same_line;
+        const uint32_t lo = min(a, bq), hi = max(a, bq);
+        const bool want = bit && a != bq;
+        uint64_t todo = __ballot(want);
+        bool leader = false;
+        while (todo)
+        {
+            const uint32_t l = static_cast<uint32_t>(__builtin_ctzll(todo));
+            const uint32_t llo = ReadLane(lo, l), lhi = ReadLane(hi, l);
+            todo &= ~__ballot(want && lo == llo && hi == lhi);
+            leader = leader || lane == l;
+        }
+        if (leader)
+            GlobalUnion(Glob(d.parent), lo, hi, d.n0 * d.n1, err);
     }
 
-    __global__ __launch_bounds__(256) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+    __global__ __launch_bounds__(128) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                    const uint32_t total_blocks)
     {
         const uint32_t b = RemapBlock(total_blocks);
@@ -691,24 +707,39 @@ namespace swmi
         const uint32_t lane = threadIdx.x & 63u;
         const uint32_t w = Uniform(threadIdx.x >> 6);
         const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw, twr = d.twr;
-        const uint32_t lx = lane & (tw - 1u);
         const gptr<uint32_t> err = Glob(d.state) + kStErr;
 
-        // right edge of tile w: lane k = lattice row k of the tile; labels from the compact edge columns
-        const uint64_t right_bits = bb[1u + w];
-        if (right_bits)
+        if (w == 0)
         {
+            // right edge: the block's lattice rows, 64 per pass (lane = row); bit r of word 1 + t is row
+            // t * twr * rpw + r; labels from the compact edge columns
+            const uint64_t r0 = bb[1], r1 = bb[2], r2 = bb[3], r3 = bb[4];
+            if ((r0 | r1 | r2 | r3) == 0ull)
+                return;
+            const uint32_t rows = bl.nwr * rpw, per_tile = twr * rpw;
             const uint32_t sx_r = (bl.sx + 1u == d.tiles_x) ? 0u : bl.sx + 1u;
             const uint32_t lb_r = bl.sy * d.tiles_x + sx_r;
-            const uint32_t row = w * twr * rpw + lane;
+            const uint32_t base_r = BlockBase(d, sx_r, bl.sy);
             const gptr<const uint16_t> ec = Glob(static_cast<const uint16_t*>(d.ecol));
-            MergeEdge(d, bl, right_bits, ec + (2u * bl.lb + 1u) * d.ecol_stride + row, ec + (2u * lb_r) * d.ecol_stride + row,
-                      BlockBase(d, sx_r, bl.sy), lane > 0, err);
+            const gptr<const uint16_t> ea = ec + (2u * bl.lb + 1u) * d.ecol_stride;
+            const gptr<const uint16_t> eb = ec + (2u * lb_r) * d.ecol_stride;
+            for (uint32_t p0 = 0; p0 < rows; p0 += 64u)
+            {
+                const uint32_t row = p0 + lane;
+                const uint32_t t = row / per_tile, r = row - t * per_tile;
+                const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
+                const uint64_t bits = __ballot(row < rows && ((word >> r) & 1ull) != 0ull);
+                if (bits)
+                    MergeEdge(d, bl, bits, ea + row, eb + row, base_r, lane > 0, err);
+            }
         }
-        // down edge: lanes of the last lattice row of the block
-        const uint64_t down_bits = (w == 0) ? bb[0] : 0ull;
-        if (down_bits)
+        else
         {
+            // down edge: lanes of the last lattice row of the block
+            const uint64_t down_bits = bb[0];
+            if (down_bits == 0ull)
+                return;
+            const uint32_t lx = lane & (tw - 1u);
             const uint32_t y = (bl.wyb + bl.nwr - 1u) * rpw + lane / tw;
             const uint32_t x = bl.sx * tw + lx;
             const bool wrap = y + 1u == n1;
@@ -1121,7 +1152,7 @@ namespace swmi
 
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
     {
-        k_merge<<<XcdGrid(total_blocks), 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+        k_merge<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
     void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
