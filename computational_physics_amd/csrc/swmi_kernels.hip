@@ -579,11 +579,13 @@ namespace swmi
         }
         __syncthreads();
 
-        // every site points straight at its block-local root (kRows chases in flight per lane); sites on
-        // the block edge mark their root
+        // every site's block-local root (kRows chases in flight per lane); sites on the block edge mark their
+        // root. A tile of at most kRows rows keeps its roots in registers for the output below; taller
+        // tiles write them back into P.
+        const bool one_chunk = nrow <= kRows;
+        uint32_t xs[kRows];
         for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
         {
-            uint32_t xs[kRows];
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
                 xs[i] = (r0 + i < nrow) ? (r_begin + r0 + i) * 64u + lane : (r_begin + r0) * 64u + lane;
@@ -605,10 +607,11 @@ namespace swmi
                 if (r0 + i < nrow)
                 {
                     const uint32_t wr = r_begin + r0 + i;
-                    P[wr * 64u + lane] = xs[i];
+                    if (!one_chunk)
+                        P[wr * 64u + lane] = xs[i];
                     const bool on_edge = lx == 0 || rbound || (wr == 0 && lrow == 0) || (wr + 1u == bl.nwr && lrow == rpw - 1u);
-                    (void)__hip_atomic_fetch_or(&F[xs[i] >> 5], on_edge ? (1u << (xs[i] & 31u)) : 0u, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (on_edge)
+                        (void)__hip_atomic_fetch_or(&F[xs[i] >> 5], 1u << (xs[i] & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }
@@ -619,11 +622,9 @@ namespace swmi
         const uint32_t cap = d.edge_cap;
         const gptr<uint16_t> elist = Glob(d.edge_list) + (kWavesPerBlock * bl.lb + w) * cap;
         uint32_t nedge = 0;
-        for (uint32_t r = 0; r < nrow; ++r)
-        {
+        const auto emit = [&](const uint32_t r, const uint32_t root) {
             const uint32_t q = (r_begin + r) * 64u + lane;
             const uint32_t site = bl.base + (r_begin + r) * bl.stride + lane;
-            const uint32_t root = LdsLoad(&P[q]);
             const bool flagged = ((LdsLoad(&F[root >> 5]) >> (root & 31u)) & 1u) != 0u;
             const uint16_t lab = static_cast<uint16_t>(root | (flagged ? kEdgeFlag : 0u));
             local[site] = lab;
@@ -639,6 +640,18 @@ namespace swmi
                     elist[pos] = static_cast<uint16_t>(q);
             }
             nedge += static_cast<uint32_t>(__popcll(K));
+        };
+        if (one_chunk)
+        {
+#pragma unroll
+            for (uint32_t i = 0; i < kRows; ++i)
+                if (i < nrow)
+                    emit(i, xs[i]);
+        }
+        else
+        {
+            for (uint32_t r = 0; r < nrow; ++r)
+                emit(r, LdsLoad(&P[(r_begin + r) * 64u + lane]));
         }
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         if (lane == 0)

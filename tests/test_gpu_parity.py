@@ -149,12 +149,14 @@ def test_run_series_and_moments(swmi, oracle):
                                rtol=1e-13)
 
 
-def test_large_lattice_one_sweep_bit_exact(swmi, oracle):
-    """4096^2 at T_c: one sweep, bonds + labels + spins bit-exact against the oracle."""
-    n0 = n1 = 4096
-    print("4096^2 one sweep", flush=True)
-    ref = oracle.PerSite(n0, n1, TC, seed=1)
-    with _engine(swmi, n0, n1, seed=1) as e:
+@pytest.mark.parametrize("n,T", [(4096, TC), (4096, 2.0), (4096, 2.6), (8192, 2.0)])
+def test_large_lattice_one_sweep_bit_exact(swmi, oracle, n, T):
+    """Full-size lattices: one sweep, bonds + labels + spins bit-exact against the oracle. T = 2.0 has one
+    giant cluster (atomic contention in the merge), T = 2.6 many small ones."""
+    n0 = n1 = n
+    print(f"{n}^2 T {T} one sweep", flush=True)
+    ref = oracle.PerSite(n0, n1, T, seed=1)
+    with _engine(swmi, n0, n1, T=T, seed=1) as e:
         e.record_bonds(True)
         e.sweep(1)
         rb, rl = ref.sweep(record=True)
@@ -188,3 +190,20 @@ def test_parity_rule_freezes_site_zero_on_the_gpu(swmi):
         for _ in range(10):
             e.sweep(3)
             assert int(e.spins()[0]) == first
+
+
+@pytest.mark.parametrize("n,T,sweeps", [(4096, 2.0, 40), (8192, TC, 10)])
+def test_large_lattice_invariants_over_sweeps(swmi, oracle, n, T, sweeps):
+    """Size-independent properties at full size after many sweeps: canonical labels (root = minimum
+    index, idempotent), the parity rule's frozen site 0, observables equal to a brute-force sum."""
+    with _engine(swmi, n, n, T=T, seed=2) as e:
+        s0 = int(e.spins()[0])
+        e.sweep(sweeps)
+        labels = e.labels()
+        spins = e.spins()
+        assert e.error_flags() == 0
+        idx = np.arange(n * n, dtype=np.uint32)
+        assert np.all(labels <= idx)
+        assert np.all(labels[labels] == labels)
+        assert int(spins[0]) == s0
+        assert e.energy_magnetization() == oracle.observables(n, n, spins)
