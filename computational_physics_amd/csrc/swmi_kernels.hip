@@ -261,11 +261,11 @@ namespace swmi
         return D & ~later;
     }
 
-    // Branch-free union of the pairs (a, b) held by the lanes, all lanes in lockstep. Each step moves
-    // both ends up (two parents, halving the path); when both are roots the larger is linked under the smaller with
-    // atomicMin, and a lost race continues with the value found (the loop of the reference Merge,
-    // swendsen_wang_amd_gpu.cpp:368-401). Lanes with nothing to do hold a == b; for them, and for ends
-    // that are not roots yet, the atomic is atomicMin(P[hi], hi), which never changes P (P[x] <= x).
+    // Union of the pairs (a, b) held by the lanes, all lanes in lockstep. Each step moves the ends that
+    // are not roots up (two parents, halving the path); when both are roots the larger is linked under the
+    // smaller with atomicMin, and a lost race continues with the value found (the loop of the reference
+    // Merge, swendsen_wang_amd_gpu.cpp:368-401). Lanes with nothing to do hold a == b; only lanes that
+    // link issue the atomic (idle lanes would all hit the same word).
     __device__ __forceinline__ void LdsUnionLockstep(uint32_t* P, uint32_t a, uint32_t b)
     {
         while (__ballot(a != b))
@@ -274,8 +274,9 @@ namespace swmi
             const uint32_t pb = LdsLoad(&P[b]);
             const bool roots = (pa == a) && (pb == b);
             const uint32_t lo = min(a, b), hi = max(a, b);
-            const uint32_t old = __hip_atomic_fetch_min(&P[hi], (roots && a != b) ? lo : hi, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            uint32_t old = hi;
+            if (roots && a != b)
+                old = __hip_atomic_fetch_min(&P[hi], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const bool linked = roots && (a == b || old == hi);
             // ends that are not roots move two steps up and point their node at the grandparent (path
             // halving, monotone like every write to P)
