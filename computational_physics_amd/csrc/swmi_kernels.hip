@@ -299,8 +299,8 @@ namespace swmi
     // DBG: the inspection modes (injected bonds instead of generated ones, bond recording) are compiled in;
     // the sweep kernel proper has neither branch.
     template <uint32_t TWR_MAX, bool TW64, bool DBG>
-    __global__ __launch_bounds__(256) void k_bond_label(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
-                                                        const uint32_t total_blocks)
+    __device__ __forceinline__ void BondLabel(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                              const uint32_t total_blocks)
     {
         // All loads of a chunk of rows are issued before its first store: on this hardware the memory
         // counter that a load waits on also counts the stores issued before it.
@@ -664,6 +664,24 @@ namespace swmi
             if (nedge > cap)
                 (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    }
+
+    // 64-wide tiles of up to 16 rows: eight workgroups per CU (the register budget of 8 waves per SIMD is
+    // asked for explicitly; the compiler keeps the excess scalar registers in vector lanes). Narrow tiles
+    // keep the default budget (the cap would spill vector registers to scratch), and taller tiles are
+    // limited by their LDS anyway.
+    template <uint32_t TWR_MAX, bool DBG>
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bond_label(
+        const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat, const uint32_t total_blocks)
+    {
+        BondLabel<TWR_MAX, true, DBG>(descs, block_lat, total_blocks);
+    }
+
+    template <uint32_t TWR_MAX, bool TW64, bool DBG>
+    __global__ __launch_bounds__(256) void k_bond_label_any(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                            const uint32_t total_blocks)
+    {
+        BondLabel<TWR_MAX, TW64, DBG>(descs, block_lat, total_blocks);
     }
 
     // =============================================================================================
@@ -1138,14 +1156,15 @@ namespace swmi
                                  const uint32_t max_twr, hipStream_t stream)
     {
         const uint32_t g = XcdGrid(total_blocks);
+        const auto launch = [&](auto kernel) { kernel<<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks); };
         if (max_twr <= 8)
-            k_bond_label<8, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            TW64 ? launch(k_bond_label<8, DBG>) : launch(k_bond_label_any<8, false, DBG>);
         else if (max_twr <= 16)
-            k_bond_label<16, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            TW64 ? launch(k_bond_label<16, DBG>) : launch(k_bond_label_any<16, false, DBG>);
         else if (max_twr <= 32)
-            k_bond_label<32, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            launch(k_bond_label_any<32, TW64, DBG>);
         else
-            k_bond_label<64, TW64, DBG><<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            launch(k_bond_label_any<64, TW64, DBG>);
     }
 
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
