@@ -156,38 +156,46 @@ namespace swmi
 
     // Parents satisfy L[x] <= x, so a chase ends within n steps; a value outside [0, n) can only come
     // from a broken invariant, which is reported through the lattice's error flag instead of being
-    // dereferenced. The first nodes of the path are pointed at the root found (path compression; the
-    // atomics return nothing, so the wave does not wait for them).
-    __device__ __forceinline__ bool GlobalFind(gptr<uint32_t> L, uint32_t& x, const uint32_t n)
-    {
-        uint32_t path[3];
-        uint32_t hops = 0;
-        uint32_t p = CachedLoad(&L[x]);
-        while (p != x)
-        {
-            if (p >= n || p > x)
-                return false;
-            if (hops < 3)
-                path[hops] = x;
-            ++hops;
-            x = p;
-            p = CachedLoad(&L[x]);
-        }
-        // path[k] (k < min(hops, 3)) is a node whose parent is not yet known to be the root x
-        for (uint32_t k = 0; k + 1 < hops && k < 3; ++k)
-            (void)__hip_atomic_fetch_min(&L[path[k]], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return true;
-    }
-
+    // dereferenced. Both ends are chased together (their loads in flight at once); a start node more
+    // than one hop from its root is pointed at the root (the atomic returns nothing, so the wave does not
+    // wait for it). Then the larger root is linked under the smaller, and a lost race continues with the
+    // value found (the loop of the reference Merge, swendsen_wang_amd_gpu.cpp:368-401).
     __device__ void GlobalUnion(gptr<uint32_t> L, uint32_t a, uint32_t b, const uint32_t n, gptr<uint32_t> err)
     {
         for (;;)
         {
-            if (a >= n || b >= n || !GlobalFind(L, a, n) || !GlobalFind(L, b, n))
+            if (a >= n || b >= n)
+                break;
+            const uint32_t a0 = a, b0 = b;
+            uint32_t pa = CachedLoad(&L[a]), pb = CachedLoad(&L[b]);
+            uint32_t ha = 0, hb = 0;
+            bool bad = false;
+            while (pa != a || pb != b)
             {
-                (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                return;
+                if (pa >= n || pa > a || pb >= n || pb > b)
+                {
+                    bad = true;
+                    break;
+                }
+                if (pa != a)
+                {
+                    a = pa;
+                    pa = CachedLoad(&L[a]);
+                    ++ha;
+                }
+                if (pb != b)
+                {
+                    b = pb;
+                    pb = CachedLoad(&L[b]);
+                    ++hb;
+                }
             }
+            if (bad)
+                break;
+            if (ha > 1)
+                (void)__hip_atomic_fetch_min(&L[a0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (hb > 1)
+                (void)__hip_atomic_fetch_min(&L[b0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (a == b)
                 return;
             if (a > b)
@@ -201,6 +209,7 @@ namespace swmi
                 return;
             b = old;
         }
+        (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
     // ---- work decomposition ----
@@ -690,18 +699,16 @@ namespace swmi
     // One two-wave workgroup per block: wavefront 0 takes the right edge (one lane per lattice row),
     // wavefront 1 the down edge (one lane per column). Two waves per block keep every block of a 4096^2
     // lattice resident at once, so the latency-bound union chains of all blocks overlap.
-    // bit `lane` of `bits`: a bond from the site labelled *pa (this block) to the site labelled *pb (block
-    // at base_b); same_line: lanes lane-1 and lane are neighbours along the edge (runs deduplicate)
-    __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const gptr<const uint16_t> pa,
-                                              const gptr<const uint16_t> pb, const uint32_t base_b, const bool same_line,
-                                              gptr<uint32_t> err)
+    // bit `lane` of `bits`: a bond from the site labelled va (this block) to the site labelled vb (block
+    // at base_b)
+    __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t va,
+                                              const uint32_t vb, const uint32_t base_b, gptr<uint32_t> err)
     {
         const uint32_t lane = threadIdx.x & 63u;
         const bool bit = LaneBit(bits);
         uint32_t a = 0, bq = 0;
         if (bit)
         {
-            const uint32_t va = *pa, vb = *pb;
             if (!(va & kEdgeFlag) || !(vb & kEdgeFlag))
                 (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a = SiteOf(bl.base, bl.stride, va & 0x7FFFu);
@@ -709,7 +716,6 @@ namespace swmi
         }
         // one union per distinct pair of roots: the lowest lane holding a pair unions it for all lanes
         // holding the same pair (below T_c most bonds of an edge join the same two clusters)
-        (void)same_line;
         const uint32_t lo = min(a, bq), hi = max(a, bq);
         const bool want = bit && a != bq;
         uint64_t todo = __ballot(want);
@@ -744,10 +750,7 @@ namespace swmi
         if (w == 0)
         {
             // right edge: the block's lattice rows, 64 per pass (lane = row); bit r of word 1 + t is row
-            // t * twr * rpw + r; labels from the compact edge columns
-            const uint64_t r0 = bb[1], r1 = bb[2], r2 = bb[3], r3 = bb[4];
-            if ((r0 | r1 | r2 | r3) == 0ull)
-                return;
+            // t * twr * rpw + r; labels from the compact edge columns, loaded with the bond words
             const uint32_t rows = bl.nwr * rpw, per_tile = twr * rpw;
             const uint32_t sx_r = (bl.sx + 1u == d.tiles_x) ? 0u : bl.sx + 1u;
             const uint32_t lb_r = bl.sy * d.tiles_x + sx_r;
@@ -757,27 +760,28 @@ namespace swmi
             const gptr<const uint16_t> eb = ec + (2u * lb_r) * d.ecol_stride;
             for (uint32_t p0 = 0; p0 < rows; p0 += 64u)
             {
-                const uint32_t row = p0 + lane;
+                const uint32_t row = min(p0 + lane, rows - 1u);
+                const uint32_t va = ea[row], vb = eb[row];
+                const uint64_t r0 = bb[1], r1 = bb[2], r2 = bb[3], r3 = bb[4];
                 const uint32_t t = row / per_tile, r = row - t * per_tile;
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
-                const uint64_t bits = __ballot(row < rows && ((word >> r) & 1ull) != 0ull);
+                const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
                 if (bits)
-                    MergeEdge(d, bl, bits, ea + row, eb + row, base_r, lane > 0, err);
+                    MergeEdge(d, bl, bits, va, vb, base_r, err);
             }
         }
         else
         {
             // down edge: lanes of the last lattice row of the block
-            const uint64_t down_bits = bb[0];
-            if (down_bits == 0ull)
-                return;
             const uint32_t lx = lane & (tw - 1u);
             const uint32_t y = (bl.wyb + bl.nwr - 1u) * rpw + lane / tw;
             const uint32_t x = bl.sx * tw + lx;
             const bool wrap = y + 1u == n1;
             const gptr<const uint16_t> loc = Glob(static_cast<const uint16_t*>(d.local));
-            MergeEdge(d, bl, down_bits, loc + y * n0 + x, loc + (wrap ? 0u : y + 1u) * n0 + x,
-                      BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u), lx != 0, err);
+            const uint32_t va = loc[y * n0 + x], vb = loc[(wrap ? 0u : y + 1u) * n0 + x];
+            const uint64_t down_bits = bb[0];
+            if (down_bits)
+                MergeEdge(d, bl, down_bits, va, vb, BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u), err);
         }
     }
 
