@@ -134,6 +134,28 @@ def batched_leg(cpa, warmup: int = 1000, sweeps: int = 5000, alone_sweeps: int =
             "workload": "256 lattices, L=2^k k~U{5..10}, T~U[1.8,2.8], default_rng(0); one engine, graph-captured"}
 
 
+def accuracy_leg(cpa, n: int = 4096, warmup: int = 1000, sweeps: int = 4000, sep: int = 10) -> dict:
+    """E/N and C/N of the plain path against the exact finite-lattice values (Ferdinand-Fisher,
+    tests/golden/exact_values.json), with binning/jackknife error bars from the on-device series; both
+    flip rules. Outside the timed region."""
+    exact = {(v["L"], v["T"]): v for v in json.loads((ROOT / "tests" / "golden" / "exact_values.json").read_text())["values"]}
+    ex = exact[(n, "2.2691853")]
+    out = {"workload": f"{n}x{n} T=2.2691853, {warmup} warmup + {sweeps} sweeps, a sample every {sep}, 20 bins",
+           "exact_e_per_spin": ex["U_per_spin"], "exact_c_per_spin": ex["C_per_spin"]}
+    for rule, name in ((0, "parity_rule"), (1, "random_rule")):
+        cfg = cpa.LatticeConfig(n, n, T_C, seed=7, max_samples=sweeps // sep, flip_rule=rule)
+        with cpa.Engine(cfg) as e:
+            e.sweep(warmup)
+            e.reset_measurements()
+            e.run(sweeps, sep)
+            eb = e.error_bars(0, 20)
+            flags = e.error_flags()
+        out[name] = {"e_per_spin": eb["e"], "e_err": eb["e_err"], "z_e": (eb["e"] - ex["U_per_spin"]) / eb["e_err"],
+                     "c_per_spin": eb["c"], "c_err": eb["c_err"], "abs_m": eb["abs_m"], "tau_int_e": eb["tau_int_e"],
+                     "samples": int(eb["samples"]), "error_flags": flags}
+    return out
+
+
 def plain_configs_leg(cpa, warmup: int = 50, sweeps: int = 300) -> dict:
     """The other plain-path configurations: 1024^2 at T_c with the default and the autotuned tile height,
     4096^2 and 8192^2 at T in {2.0, 2.2691853, 2.6}; sweep-only rates and the 4096^2 rate of the reference
@@ -274,6 +296,9 @@ def main() -> int:
     }
     log(f"plain {n}x{n}: {value:.3e} spin-updates/s, {ms_per_step * 1e3:.1f} us/sweep")
     if not args.no_configs:
+        result["accuracy"] = accuracy_leg(cpa)
+        log(f"accuracy: z(E/N) parity {result['accuracy']['parity_rule']['z_e']:.2f}, "
+            f"random {result['accuracy']['random_rule']['z_e']:.2f}")
         result["plain_configs"] = plain_configs_leg(cpa)
         log("plain configs done")
     if not args.no_batched:

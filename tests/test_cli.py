@@ -37,6 +37,7 @@ def test_help_lists_reference_flags():
     (["--target=cpu"], "Unknown target device."),
     (["--rng=mt19937"], "Unknown RNG name."),
     (["--algorithm=metropolis"], "Unknown algorithm."),
+    (["--flip_rule=coin"], "Unknown flip rule."),
 ])
 def test_errors_like_reference(args, msg):
     r = run(*args)
@@ -83,3 +84,17 @@ def test_cli_matches_python_engine(swmi):
     t = np.float32(2.2691853142130221)
     c = (mom[2] / mom[0] - (mom[1] / mom[0]) ** 2) * 4096 / float(t * t)
     assert rep["c"] == pytest.approx(c, rel=1e-10)
+
+
+@pytest.mark.gpu
+def test_cli_random_flip_rule_matches_python_engine(swmi):
+    r = run("--extent=64x64", "--num_warmup=50", "--num_sweeps=300", "--seed=4", "--flip_rule=random")
+    assert r.returncode == 0, r.stderr
+    rep = report(r.stdout)
+    with swmi.Engine(swmi.LatticeConfig(64, 64, seed=4, flip_rule=swmi.FLIP_RANDOM)) as e:
+        e.sweep(50)
+        e.reset_measurements()
+        e.run(300, 10)
+        mom = e.moments()
+    assert rep["e"] == pytest.approx(mom[1] / mom[0], rel=1e-13)
+    assert rep["m"] == pytest.approx(mom[3] / mom[0], rel=1e-13)
