@@ -22,6 +22,8 @@
  *                           checked against a full or_lcg32 per group
  *   or_murmur3_32/or_fmix32 MurmurHash3_x86_32 published test vectors (SMHasher); or_flip_random is
  *                           their composition with the pinned labeller
+ *   or_refcpu_measure       report lines of the compiled reference driver (tests/golden/protocol.json)
+ *                           and the reference's published sample (README.md:66-82)
  */
 #include "oracle.h"
 
@@ -689,4 +691,36 @@ void or_ps_sweep_rule(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spi
     free(u1);
     free(bonds);
     free(labels);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * The reference driver's measurement protocol (ising/src/ising_2d.cpp:122-162) on the restated CPU
+ * path: warmup sweeps, then n_sep sweeps and a sample (e = -E/N, m = M/N, lattice.cpp:29-67) while
+ * i < sweeps; FP64 sums of e, e^2 and |m|; C = (<e^2> - <e>^2) * N / (T*T with T float).
+ * out = {<e>, C, <|m|>}.
+ * ------------------------------------------------------------------------------------------------ */
+void or_refcpu_measure(or_refcpu* r, uint8_t* spins, float temperature, uint32_t warmup, uint32_t sweeps, uint32_t n_sep,
+                       double* out)
+{
+    const uint32_t n0 = r->n0, n1 = r->n1;
+    const double n = (double)n0 * (double)n1;
+    for (uint32_t i = 0; i < warmup; ++i)
+        or_refcpu_sweep(r, spins, temperature, NULL, NULL);
+    double energy = 0.0, energy_squared = 0.0, magnetization = 0.0;
+    for (uint32_t i = 0; i < sweeps; i += n_sep)
+    {
+        for (uint32_t k = 0; k < n_sep; ++k)
+            or_refcpu_sweep(r, spins, temperature, NULL, NULL);
+        int64_t e = 0, m = 0;
+        or_observables(n0, n1, spins, &e, &m);
+        const double ev = -1.0 * (double)e / n, mv = 1.0 * (double)m / n;
+        energy += ev;
+        energy_squared += ev * ev;
+        magnetization += fabs(mv);
+    }
+    const double samples = (double)(sweeps / n_sep);
+    const float t2 = temperature * temperature;
+    out[0] = energy / samples;
+    out[1] = ((energy_squared / samples) - (energy / samples) * (energy / samples)) * n / (double)t2;
+    out[2] = magnetization / samples;
 }

@@ -75,6 +75,9 @@ or_refcpu* or_refcpu_create(uint32_t n0, uint32_t n1, uint32_t w, uint32_t num_t
 void or_refcpu_destroy(or_refcpu* r);
 /* One sweep. bonds_out / labels_out may be NULL. */
 void or_refcpu_sweep(or_refcpu* r, uint8_t* spins, float temperature, uint8_t* bonds_out, uint32_t* labels_out);
+/* the reference driver's measurement loop (ising_2d.cpp:122-162): out = {<e>, C/N, <|m|>} */
+void or_refcpu_measure(or_refcpu* r, uint8_t* spins, float temperature, uint32_t warmup, uint32_t sweeps, uint32_t n_sep,
+                       double* out);
 
 /* ---- the per-site lcg32 layout of the MI355X engine ----
  * Group g = sites [64g, 64g+64) in row-major order; lane i of the group = site 64g+i. Each group is one

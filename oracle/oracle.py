@@ -64,6 +64,7 @@ def lib() -> ctypes.CDLL:
     L.or_flip_key.restype = u32
     L.or_flip_random.argtypes = [ctypes.c_size_t, vp, vp, u32]
     L.or_observables.argtypes = [u32, u32, vp, P(ctypes.c_int64), P(ctypes.c_int64)]
+    L.or_refcpu_measure.argtypes = [vp, vp, ctypes.c_float, u32, u32, u32, P(ctypes.c_double)]
     L.or_refcpu_create.argtypes = [u32] * 6
     L.or_refcpu_create.restype = vp
     L.or_refcpu_destroy.argtypes = [vp]
@@ -141,6 +142,12 @@ class RefCpu:
 
     def sweep(self, spins: np.ndarray, temperature: float) -> None:
         lib().or_refcpu_sweep(self._h, spins.ctypes.data, temperature, None, None)
+
+    def measure(self, spins: np.ndarray, temperature: float, warmup: int, sweeps: int, n_sep: int = 10):
+        """The reference driver's measurement loop; returns (<e>, C/N, <|m|>)."""
+        out = (ctypes.c_double * 3)()
+        lib().or_refcpu_measure(self._h, spins.ctypes.data, temperature, warmup, sweeps, n_sep, out)
+        return tuple(out[:])
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -11,6 +11,8 @@ outputs as data:
   refcpu.npz            spin snapshots + (E, M) sums of the reference CPU path (Lattice<2>::Update with
                         SwendsenWang_2D<LCG32, CPU>) after given sweep counts, several extents,
                         temperatures and NUM_THREADS
+  protocol.json         the report lines (E/N, C/N, |m|) of the compiled reference driver Ising2D
+                        (ising/src/ising_2d.cpp: 1000 warmup sweeps, a sample every 10) for short runs
 
     python tests/golden/make_golden.py
 """
@@ -34,6 +36,8 @@ UPDATES = 61
 BUFFERED_SIZES = [16, 24, 16, 5, 40, 24, 1, 64, 17]
 TEMPERATURES = ["2.0", "2.2691853", "2.6", "1.5", "1.8", "2.8", "3.5", "0.5", "10.0", "2.2691853142130221"]
 # (n0, n1, T, NUM_THREADS, sweep counts)
+# (NUM_THREADS, n0, n1, T, measured sweeps) for the driver's report lines
+PROTOCOL_CASES = [(32, 128, 128, "2.2691853", 3000), (3, 64, 32, "2.0", 2000), (5, 96, 64, "2.6", 2000)]
 REFCPU_CASES = [
     (32, 32, "2.2691853", 1, [0, 1, 2, 5, 20]),
     (64, 48, "2.0", 4, [0, 1, 3, 10]),
@@ -88,6 +92,21 @@ def main() -> int:
             meta.append({"name": name, "n0": n0, "n1": n1, "temperature": t, "num_threads": nthreads, "sweeps": ks})
     arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(HERE / "refcpu.npz", **arrays)
+
+    # report lines of the reference driver for short runs of its measurement protocol
+    cases = []
+    for nthreads, n0, n1, t, sweeps in PROTOCOL_CASES:
+        env = dict(os.environ, NUM_THREADS=str(nthreads))
+        out = run([str(REF / "Ising2D"), f"--extent={n0}x{n1}", "--target=cpu", f"--temperature={t}",
+                   f"--num_sweeps={sweeps}"], env=env)
+        vals = {}
+        for key, name in (("Internal energy per spin", "e"), ("Heat capacity per spin", "c"),
+                          ("Absolute Magnetization per spin", "m")):
+            line = next(l for l in out.splitlines() if l.startswith(key + ":"))
+            vals[name] = line.split(":", 1)[1].strip()
+        cases.append({"num_threads": nthreads, "n0": n0, "n1": n1, "temperature": t, "sweeps": sweeps, "report": vals})
+    (HERE / "protocol.json").write_text(json.dumps({"source": "oracle/_ref/Ising2D (reference sources, reference flags)",
+                                                    "cases": cases}, indent=1) + "\n")
     print("wrote", sorted(p.name for p in HERE.glob("*") if p.suffix in (".npz", ".json")))
     return 0
 

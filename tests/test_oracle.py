@@ -276,3 +276,28 @@ def test_random_flip_flips_whole_clusters_half_of_the_time(oracle):
     # site 0 is not frozen under this rule: over keys its cluster flips about half of the time
     hits = [oracle.fmix32(int(labels[0]) ^ oracle.flip_key(3, 2 * k)) >> 31 for k in range(400)]
     assert 0.4 < np.mean(hits) < 0.6
+
+
+PROTOCOL = json.loads((GOLDEN / "protocol.json").read_text())
+
+
+@pytest.mark.parametrize("case", PROTOCOL["cases"], ids=lambda c: f"{c['n0']}x{c['n1']}-T{c['temperature']}-t{c['num_threads']}")
+def test_reference_driver_protocol(oracle, case):
+    """The restated reference CPU path under the reference driver's measurement loop (1000 warmup sweeps,
+    a sample every 10, FP64 sums) prints the same report lines as the compiled reference Ising2D."""
+    n0, n1 = case["n0"], case["n1"]
+    r = oracle.RefCpu(n0, n1, 16, case["num_threads"])
+    s = oracle.hot_start(n0 * n1, 1)
+    t = float(np.float32(float(case["temperature"])))
+    e, c, m = r.measure(s, t, 1000, case["sweeps"], 10)
+    assert ("%g" % e, "%g" % c, "%g" % m) == (case["report"]["e"], case["report"]["c"], case["report"]["m"])
+
+
+@pytest.mark.skipif(os.environ.get("SWMI_SLOW_TESTS") != "1", reason="3 minutes on one core; set SWMI_SLOW_TESTS=1")
+def test_reference_readme_sample(oracle):
+    """The reference's published sample (README.md:66-82): NUM_THREADS=32, 128x128 at T_c, 1000 warmup +
+    100000 sweeps on its CPU path prints E/N -1.41921, C/N 2.54411, |m| 0.550833."""
+    r = oracle.RefCpu(128, 128, 16, 32)
+    s = oracle.hot_start(128 * 128, 1)
+    e, c, m = r.measure(s, float(np.float32(2.2691853142130221)), 1000, 100000, 10)
+    assert ("%g" % e, "%g" % c, "%g" % m) == ("-1.41921", "2.54411", "0.550833")
