@@ -259,6 +259,12 @@ bool swmi_server::Service(Lane& lane)
         }
         lane.active.swap(keep);
     }
+    {
+        // shutting down: no new chunk; resident jobs end unfinished (their results are not reported)
+        std::lock_guard<std::mutex> lock(mu);
+        if (stop)
+            return progressed;
+    }
     progressed = Admit(lane) || progressed;
     if (lane.dirty)
     {

@@ -241,7 +241,8 @@ typedef struct
 } swmi_server_stats;
 
 int swmi_server_create(const swmi_server_config* config, swmi_server** out);
-/* Finishes the chunks in flight, drops queued jobs and frees the device memory. */
+/* Waits for the chunks in flight, then drops queued and resident jobs (without results) and frees the
+ * device memory. */
 int swmi_server_destroy(swmi_server* server);
 /* Thread-safe; returns immediately. */
 int swmi_server_submit(swmi_server* server, const swmi_job* job, uint64_t* job_id);

@@ -199,3 +199,13 @@ def test_batch_autotune_returns_a_height_and_results_do_not_depend_on_it(swmi):
         b.sweep(12)
         for i in range(len(cfgs)):
             np.testing.assert_array_equal(a.spins(i), b.spins(i), err_msg=f"lattice {i}")
+
+
+def test_server_destroy_does_not_wait_for_long_jobs(swmi):
+    """Destroying a server with a long resident job returns after the chunk in flight, not the job."""
+    srv = swmi.Server(chunk_sweeps=10)
+    srv.submit(swmi.Job(1024, 1024, TC, seed=1, sweeps=10 ** 6))   # about 30 s of device time
+    time.sleep(0.2)
+    t0 = time.perf_counter()
+    srv.close()
+    assert time.perf_counter() - t0 < 5.0
