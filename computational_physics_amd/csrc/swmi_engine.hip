@@ -210,9 +210,32 @@ swmi_engine::~swmi_engine()
 {
     if (stream)
         (void)hipStreamSynchronize(stream);
+    for (auto s : side_streams)
+        (void)hipStreamSynchronize(s);
     DropGraphs();
+    for (auto s : side_streams)
+        (void)hipStreamDestroy(s);
+    for (auto e : join_events)
+        (void)hipEventDestroy(e);
+    if (fork_event)
+        (void)hipEventDestroy(fork_event);
     if (stream)
         (void)hipStreamDestroy(stream);
+}
+
+void swmi_engine::EnsureSideStreams()
+{
+    if (!fork_event && classes.size() > 1)
+        Check(hipEventCreateWithFlags(&fork_event, hipEventDisableTiming), "event create");
+    while (side_streams.size() + 1 < classes.size())
+    {
+        hipStream_t s = nullptr;
+        Check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        side_streams.push_back(s);
+        hipEvent_t e = nullptr;
+        Check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+        join_events.push_back(e);
+    }
 }
 
 Lattice& swmi_engine::At(uint32_t i)
@@ -325,20 +348,37 @@ void swmi_engine::Rebuild()
         Check(hipMemcpyAsync(k.d_block_lat.ptr, k.block_lat.data(), k.block_lat.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                              stream),
               "work list upload");
+    EnsureSideStreams();
 }
 
 void swmi_engine::LaunchCheck(const char* what) { Check(hipGetLastError(), what); }
 
 void swmi_engine::EnqueueSweeps(uint32_t n)
 {
-    for (uint32_t i = 0; i < n; ++i)
+    if (n == 0 || classes.empty())
+        return;
+    const size_t nc = classes.size();
+    if (nc > 1)
     {
-        for (auto& k : classes)
-            LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, recording, stream);
-        for (auto& k : classes)
-            LaunchMerge(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, stream);
-        for (auto& k : classes)
-            LaunchFlip(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, kFlip | kAdvance, stream);
+        Check(hipEventRecord(fork_event, stream), "fork");
+        for (size_t c = 1; c < nc; ++c)
+            Check(hipStreamWaitEvent(side_streams[c - 1], fork_event, 0), "fork");
+    }
+    for (size_t c = 0; c < nc; ++c)
+    {
+        const KernelClass& k = classes[c];
+        hipStream_t s = ClassStream(c);
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, recording, s);
+            LaunchMerge(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
+            LaunchFlip(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, kFlip | kAdvance, s);
+        }
+    }
+    for (size_t c = 1; c < nc; ++c)
+    {
+        Check(hipEventRecord(join_events[c - 1], side_streams[c - 1]), "join");
+        Check(hipStreamWaitEvent(stream, join_events[c - 1], 0), "join");
     }
     LaunchCheck("sweep launch");
 }

@@ -181,6 +181,11 @@ struct swmi_engine
     bool recording = false;   // some lattice records its bonds (selects the inspection build of k_bond_label)
     bool use_graphs = true;
     std::vector<swmi::GraphEntry> graphs;   // valid for the current work lists only
+    // kernel classes are independent: class c > 0 runs on side stream c - 1, forked from and joined
+    // back into the engine stream around every sweep loop (parallel branches in a captured graph)
+    std::vector<hipStream_t> side_streams;
+    std::vector<hipEvent_t> join_events;
+    hipEvent_t fork_event = nullptr;
     // single-lattice work lists for the inspection hooks
     swmi::DeviceBuffer<swmi::LatDesc> d_one_desc;
     swmi::DeviceBuffer<uint32_t> d_zero_blocks;
@@ -200,6 +205,9 @@ struct swmi_engine
     void LaunchCheck(const char* what);
     void EnqueueSweeps(uint32_t n);
     void EnqueueSample(uint32_t mode);
+    // side streams and events for every class beyond the first
+    void EnsureSideStreams();
+    hipStream_t ClassStream(size_t c) const { return c == 0 ? stream : side_streams[c - 1]; }
     // n sweeps (with a sample of `sample_mode` after every `sep` sweeps when sample_mode >= 0), through
     // captured graphs when enabled.
     void Run(uint32_t n, uint32_t sep, int32_t sample_mode);
