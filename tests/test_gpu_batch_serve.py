@@ -209,3 +209,73 @@ def test_server_destroy_does_not_wait_for_long_jobs(swmi):
     t0 = time.perf_counter()
     srv.close()
     assert time.perf_counter() - t0 < 5.0
+
+
+def test_two_engines_on_two_threads_equal_sequential(swmi):
+    """Engines are independent: two host threads driving two engines at once give the results of running
+    them one after the other."""
+    import threading
+    cfgs = [_cfg(swmi, 256, 256, 2.1, seed=50), _cfg(swmi, 512, 128, 2.4, seed=51)]
+    out = [None, None]
+
+    def work(i):
+        with swmi.Engine(cfgs[i]) as e:
+            e.run(300, 10)
+            out[i] = (e.spins(), e.moments())
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for i, c in enumerate(cfgs):
+        with swmi.Engine(c) as e:
+            e.run(300, 10)
+            np.testing.assert_array_equal(e.spins(), out[i][0])
+            np.testing.assert_array_equal(e.moments(), out[i][1])
+
+
+def test_server_many_small_jobs_from_two_threads(swmi):
+    """200 small jobs submitted from two threads while earlier ones run: all finish with the plain-path
+    results (checked for a sample) and no job is lost or duplicated."""
+    import threading
+    jobs = [swmi.Job(64 * (1 + k % 3), 64, 1.8 + 0.01 * k, seed=100 + k, warmup=10 * (k % 4), sweeps=50 + 10 * (k % 7), tag=k)
+            for k in range(200)]
+    with swmi.Server() as srv:
+        def submit(part):
+            for j in part:
+                srv.submit(j)
+        threads = [threading.Thread(target=submit, args=(jobs[i::2],)) for i in range(2)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        assert srv.wait(120000) == 0
+        res = []
+        while True:
+            got = srv.poll(64)
+            if not got:
+                break
+            res.extend(got)
+        st = srv.stats()
+    assert sorted(r["tag"] for r in res) == list(range(200))
+    assert st["submitted"] == st["finished"] == 200 and st["active"] == 0
+    by_tag = {r["tag"]: r for r in res}
+    for k in (0, 37, 199):
+        r = by_tag[k]
+        assert r["status"] == 0 and r["sweeps_done"] == jobs[k].warmup + jobs[k].sweeps
+        if jobs[k].sweeps % 10 == 0:
+            mom, em = _plain_job_result(swmi, jobs[k])
+            np.testing.assert_array_equal(r["moments"], mom)
+            assert (r["energy"], r["magnetization"]) == em
+
+
+@pytest.mark.parametrize("bad", [dict(n0=63), dict(sweeps=0), dict(warmup=15), dict(temperature=-1.0),
+                                 dict(n0=65536, n1=65536)])
+def test_server_rejects_invalid_jobs(swmi, bad):
+    base = dict(n0=64, n1=64, temperature=2.0, seed=1, warmup=10, sweeps=100)
+    base.update(bad)
+    with swmi.Server() as srv:
+        with pytest.raises(swmi.SwmiError):
+            srv.submit(swmi.Job(**base))
+        assert srv.stats()["submitted"] == 0
