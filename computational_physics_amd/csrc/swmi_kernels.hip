@@ -279,24 +279,31 @@ namespace swmi
     {
         while (__ballot(a != b))
         {
+            // lanes that are done (a == b) only read P[a]; they follow no pointer and write nothing, so
+            // their a may name a node another wave has not written yet
+            const bool act = a != b;
             const uint32_t pa = LdsLoad(&P[a]);
             const uint32_t pb = LdsLoad(&P[b]);
             const bool roots = (pa == a) && (pb == b);
+            const bool up = act && !roots;
             const uint32_t lo = min(a, b), hi = max(a, b);
             uint32_t old = hi;
-            if (roots && a != b)
+            if (act && roots)
                 old = __hip_atomic_fetch_min(&P[hi], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const bool linked = roots && (a == b || old == hi);
+            const bool linked = roots && old == hi;
             // ends that are not roots move two steps up and point their node at the grandparent (path
             // halving, monotone like every write to P)
-            const uint32_t ga = roots ? 0u : LdsLoad(&P[pa]);
-            const uint32_t gb = roots ? 0u : LdsLoad(&P[pb]);
-            if (!roots && ga != pa)
+            const uint32_t ga = up ? LdsLoad(&P[pa]) : 0u;
+            const uint32_t gb = up ? LdsLoad(&P[pb]) : 0u;
+            if (up && ga != pa)
                 LdsMin(&P[a], ga);
-            if (!roots && gb != pb)
+            if (up && gb != pb)
                 LdsMin(&P[b], gb);
-            a = roots ? (linked ? hi : lo) : ga;
-            b = roots ? (linked ? hi : old) : gb;
+            if (act)
+            {
+                a = roots ? (linked ? hi : lo) : ga;
+                b = roots ? (linked ? hi : old) : gb;
+            }
         }
     }
 
@@ -402,6 +409,7 @@ namespace swmi
 
         uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
         uint32_t qlen = 0;
+        uint32_t Rp = 0;   // 64-wide tiles: root of this lane's site in the previous row
         if (nrow > 0)
         {
             // spins below the tile (down halo), needed last but fetched first
@@ -501,33 +509,78 @@ namespace swmi
                         if (r == 0)
                             s_first_B[w] = B;
 
-                        // every site starts under the first site of its row run
                         const uint64_t below = ~B & lanes_below;
                         const uint32_t run_start = below ? 64u - static_cast<uint32_t>(__clzll(below)) : 0u;
-                        P[wr * 64u + lane] = wr * 64u + run_start;
-
-                        if (r > 0)
+                        const uint32_t row_base = wr * 64u;
+                        const uint32_t q0 = row_base + run_start;
+                        if (TW64)
                         {
-                            // bonds (x, y) -> (x, y+1) of the previous wave row: queue one union per
-                            // overlap interval
-                            const uint64_t SD = TW64 ? S : ((Shr(prev_S, tw) & low) | Shl(S, 64u - tw));
-                            const uint64_t D = injected ? prev_U2 : (~(prev_S ^ SD) & prev_U2);
-                            const uint64_t tgtB = TW64 ? B : (Shr(prev_B, tw) | Shl(B, 64u - tw));
-                            const uint64_t K = FirstBondPerInterval(D, prev_B & tgtB, rowstart);
-                            if (LaneBit(K))
+                            // Previous-row roots are kept resolved in Rp, so a down bond usually needs no
+                            // union: a run's label is the minimum of its first site and the roots its down
+                            // bonds reach (one LDS atomicMin per bonded lane into the run's slot of M). Only a
+                            // bond to a root other than that minimum merges two clusters, through the
+                            // union-find. Every site then points at its root as of this row.
+                            uint32_t root = q0;
+                            if (r > 0)
                             {
-                                const uint32_t top = (wr - 1u) * 64u + lane;
-                                const uint32_t bot = TW64 ? wr * 64u + run_start
-                                                          : ((lane < 64u - tw) ? top + tw : wr * 64u + lane - (64u - tw));
-                                const uint32_t pos = qlen + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
-                                                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
-                                Qw[pos] = (top << 16) | bot;
+                                const uint64_t D = injected ? prev_U2 : (~(prev_S ^ S) & prev_U2);
+                                if (D != 0ull)
+                                {
+                                    uint32_t* const M = Qw;
+                                    if (run_start == lane)
+                                        M[lane] = q0;
+                                    const bool bonded = LaneBit(D);
+                                    if (bonded)
+                                        LdsMin(&M[run_start], Rp);
+                                    const uint32_t m = LdsLoad(&M[run_start]);
+                                    root = m;
+                                    const bool merge = bonded && Rp != m;
+                                    if (__ballot(merge))
+                                    {
+                                        LdsUnionLockstep(P, merge ? Rp : 0u, merge ? m : 0u);
+                                        // labels of this row may have been linked under another root; sites
+                                        // of this row are still roots (first sites of runs without bonds up)
+                                        for (;;)
+                                        {
+                                            const uint32_t pr = root >= row_base ? root : LdsLoad(&P[root]);
+                                            if (!__ballot(pr != root))
+                                                break;
+                                            root = pr;
+                                        }
+                                    }
+                                }
+                                if (dump)
+                                    dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
                             }
-                            qlen += static_cast<uint32_t>(__popcll(K));
-                            if (qlen >= 64u)
-                                drain64(qlen);
-                            if (dump)
-                                dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
+                            P[row_base + lane] = root;
+                            Rp = root;
+                        }
+                        else
+                        {
+                            // every site starts under the first site of its row run
+                            P[row_base + lane] = q0;
+                            if (r > 0)
+                            {
+                                // bonds of the previous wave row to the row below: queue one union per overlap
+                                // interval
+                                const uint64_t SD = (Shr(prev_S, tw) & low) | Shl(S, 64u - tw);
+                                const uint64_t D = injected ? prev_U2 : (~(prev_S ^ SD) & prev_U2);
+                                const uint64_t tgtB = Shr(prev_B, tw) | Shl(B, 64u - tw);
+                                const uint64_t K = FirstBondPerInterval(D, prev_B & tgtB, rowstart);
+                                if (LaneBit(K))
+                                {
+                                    const uint32_t top = (wr - 1u) * 64u + lane;
+                                    const uint32_t bot = (lane < 64u - tw) ? top + tw : wr * 64u + lane - (64u - tw);
+                                    const uint32_t pos = qlen + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
+                                                                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
+                                    Qw[pos] = (top << 16) | bot;
+                                }
+                                qlen += static_cast<uint32_t>(__popcll(K));
+                                if (qlen >= 64u)
+                                    drain64(qlen);
+                                if (dump)
+                                    dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
+                            }
                         }
                         prev_S = S;
                         prev_U2 = U2;
