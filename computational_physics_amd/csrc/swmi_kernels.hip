@@ -95,7 +95,14 @@ namespace swmi
         return Fmix32(h ^ 4u);
     }
 
-    // ---- union-find in LDS (parents satisfy P[x] <= x; the workgroup shares one array) ----
+    // ---- union-find in LDS. Nodes are named by their byte offset into the workgroup's parent array P
+    // (so no instruction scales an index into an address); parents satisfy P[x] <= x, so roots are cluster
+    // minima. ----
+    __device__ __forceinline__ uint32_t* At(uint32_t* P, const uint32_t off)
+    {
+        return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(P) + off);
+    }
+
     __device__ __forceinline__ uint32_t LdsLoad(const uint32_t* p)
     {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -104,44 +111,6 @@ namespace swmi
     __device__ __forceinline__ void LdsMin(uint32_t* p, const uint32_t v)
     {
         (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-
-    // find with path halving (monotone: the shortcut is written with atomicMin)
-    __device__ __forceinline__ uint32_t LdsFind(uint32_t* P, uint32_t x)
-    {
-        for (;;)
-        {
-            const uint32_t p = LdsLoad(&P[x]);
-            if (p == x)
-                return x;
-            const uint32_t g = LdsLoad(&P[p]);
-            if (g == p)
-                return p;
-            LdsMin(&P[x], g);
-            x = g;
-        }
-    }
-
-    // Same loop as the reference Merge (swendsen_wang_amd_gpu.cpp:368-401), started from roots.
-    __device__ void LdsUnion(uint32_t* P, uint32_t a, uint32_t b)
-    {
-        for (;;)
-        {
-            a = LdsFind(P, a);
-            b = LdsFind(P, b);
-            if (a == b)
-                return;
-            if (a > b)
-            {
-                const uint32_t t = a;
-                a = b;
-                b = t;
-            }
-            const uint32_t old = __hip_atomic_fetch_min(&P[b], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == b)
-                return;
-            b = old;
-        }
     }
 
     // ---- union-find in HBM over the roots of edge-reaching clusters. Links are made with agent-scope
@@ -274,7 +243,7 @@ namespace swmi
     // are not roots up (two parents, halving the path); when both are roots the larger is linked under the
     // smaller with atomicMin, and a lost race continues with the value found (the loop of the reference
     // Merge, swendsen_wang_amd_gpu.cpp:368-401). Lanes with nothing to do hold a == b; only lanes that
-    // link issue the atomic (idle lanes would all hit the same word).
+    // link issue the atomic (idle lanes would all hit the same word). a and b are byte offsets into P.
     __device__ __forceinline__ void LdsUnionLockstep(uint32_t* P, uint32_t a, uint32_t b)
     {
         while (__ballot(a != b))
@@ -282,23 +251,23 @@ namespace swmi
             // lanes that are done (a == b) only read P[a]; they follow no pointer and write nothing, so
             // their a may name a node another wave has not written yet
             const bool act = a != b;
-            const uint32_t pa = LdsLoad(&P[a]);
-            const uint32_t pb = LdsLoad(&P[b]);
+            const uint32_t pa = LdsLoad(At(P, a));
+            const uint32_t pb = LdsLoad(At(P, b));
             const bool roots = (pa == a) && (pb == b);
             const bool up = act && !roots;
             const uint32_t lo = min(a, b), hi = max(a, b);
             uint32_t old = hi;
             if (act && roots)
-                old = __hip_atomic_fetch_min(&P[hi], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                old = __hip_atomic_fetch_min(At(P, hi), lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const bool linked = roots && old == hi;
             // ends that are not roots move two steps up and point their node at the grandparent (path
             // halving, monotone like every write to P)
-            const uint32_t ga = up ? LdsLoad(&P[pa]) : 0u;
-            const uint32_t gb = up ? LdsLoad(&P[pb]) : 0u;
+            const uint32_t ga = up ? LdsLoad(At(P, pa)) : 0u;
+            const uint32_t gb = up ? LdsLoad(At(P, pb)) : 0u;
             if (up && ga != pa)
-                LdsMin(&P[a], ga);
+                LdsMin(At(P, a), ga);
             if (up && gb != pb)
-                LdsMin(&P[b], gb);
+                LdsMin(At(P, b), gb);
             if (act)
             {
                 a = roots ? (linked ? hi : lo) : ga;
@@ -309,9 +278,16 @@ namespace swmi
 
     // Per wavefront (tile) the rows are processed top to bottom with the loads of the next kCh rows in
     // flight together. Bonds are computed on wave-uniform 64-bit masks: S (spin up), U1/U2 (draw below
-    // threshold), B (bond to the right neighbour inside the tile), D (bond to the site below). Rows
-    // only record their unions (one per overlap interval) in a per-wave LDS queue, which is worked off
-    // 64 unions at a time whenever it fills (unions only involve rows already initialised).
+    // threshold), B (bond to the right neighbour inside the tile), D (bond to the site below).
+    // 64-wide tiles keep the previous row's roots resolved in registers (Rp). A row run's label is the
+    // minimum of its first site and the roots its down bonds reach (one LDS atomicMin per bonded lane into
+    // the run's slot); a bond to any other root links that root under the minimum with one atomicMin, and
+    // only a link that loses a race goes through the lockstep union.
+    // Narrow tiles record one union per overlap interval of consecutive rows in a per-wave LDS queue,
+    // worked off 64 at a time whenever it fills.
+    // Nodes of the LDS union-find are byte offsets into P. After the final find, the root of every
+    // cluster that reaches the block edge carries kPFlag in its own entry, so a site's label (root index,
+    // plus kEdgeFlag) is P[root] >> 2.
     // DBG: the inspection modes (injected bonds instead of generated ones, bond recording) are compiled in;
     // the sweep kernel proper has neither branch.
     template <uint32_t TWR_MAX, bool TW64, bool DBG>
@@ -323,9 +299,12 @@ namespace swmi
         constexpr uint32_t kCh = TWR_MAX < 16 ? TWR_MAX : 16;
         constexpr uint32_t kRows = TWR_MAX < 16 ? TWR_MAX : 16;   // rows resolved together at the end
         constexpr uint32_t kSites = kWavesPerBlock * TWR_MAX * 64;
+        constexpr uint32_t kQw = TW64 ? 64u : 128u;                // per-wave LDS words of Q
+        constexpr uint32_t kPFlag = 1u << 17;
+        static_assert(kSites * 4u <= (1u << 16), "byte offsets must fit the 16-bit halves of a queue entry");
+        static_assert((kPFlag >> 2) == kEdgeFlag, "a flagged root entry shifts into a flagged label");
         __shared__ uint32_t P[kSites];
-        __shared__ uint32_t Q[kWavesPerBlock * 128];   // per-wave union queues: (a << 16) | b
-        __shared__ uint32_t F[kSites / 32];     // roots of clusters that reach the block edge
+        __shared__ uint32_t Q[kWavesPerBlock * kQw];   // 64-wide: run minima of the current row; narrow: union queue
         __shared__ uint64_t s_first_B[kWavesPerBlock];
         __shared__ uint64_t AC[8];              // Numerical Recipes pairs (c << 32 | a) by selector
 
@@ -341,6 +320,7 @@ namespace swmi
         const LatDesc& d = descs[block_lat[b]];
         const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t lane4 = lane * 4u;
         const uint32_t w = Uniform(threadIdx.x >> 6);
 
         const uint32_t n0 = d.n0, n1 = d.n1, twr = d.twr;
@@ -374,14 +354,14 @@ namespace swmi
         const gptr<uint32_t> rotp = Glob(d.rot);
         const gptr<const uint8_t> bond_in = Glob(d.bond_in);
         const gptr<uint16_t> local = Glob(d.local);
-        const gptr<uint16_t> ecol = Glob(d.ecol);
         const gptr<uint32_t> parent = Glob(d.parent);
 
         const uint32_t r_begin = w * twr;
         const uint32_t nrow = bl.nwr > r_begin ? min(twr, bl.nwr - r_begin) : 0u;
         const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
         const bool last_wave = w + 1u == nwaves;
-        uint32_t* const Qw = Q + w * 128u;
+        uint32_t* const Qw = Q + w * kQw;
+        uint32_t* const Pl = At(P, lane4);   // this lane's entry of wave row 0
         // a full queue (>= 64 entries) is worked off right away; the tail moves to the front
         const auto drain64 = [&](uint32_t& qlen) {
             const uint32_t e = Qw[lane];
@@ -403,9 +383,7 @@ namespace swmi
         const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
         const uint64_t lanes_below = (1ull << lane) - 1ull;
 
-        for (uint32_t i = threadIdx.x; i < kSites / 32; i += 256u)
-            F[i] = 0;
-        __syncthreads();
+        __syncthreads();   // AC
 
         uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
         uint32_t qlen = 0;
@@ -509,56 +487,54 @@ namespace swmi
                         if (r == 0)
                             s_first_B[w] = B;
 
-                        const uint64_t below = ~B & lanes_below;
-                        const uint32_t run_start = below ? 64u - static_cast<uint32_t>(__clzll(below)) : 0u;
-                        const uint32_t row_base = wr * 64u;
-                        const uint32_t q0 = row_base + run_start;
+                        const uint32_t row_b = wr * 256u;   // byte offset of the wave row in P
                         if (TW64)
                         {
-                            // Previous-row roots are kept resolved in Rp, so a down bond usually needs no
-                            // union: a run's label is the minimum of its first site and the roots its down
-                            // bonds reach (one LDS atomicMin per bonded lane into the run's slot of M). Only a
-                            // bond to a root other than that minimum merges two clusters, through the
-                            // union-find. Every site then points at its root as of this row.
-                            uint32_t root = q0;
-                            if (r > 0)
+                            // slot of this lane's row run: the number of run starts after lane 0 up to this
+                            // lane; the first lane of the run initialises it with its own node
+                            uint32_t* const Mk = Qw + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(~B >> 32),
+                                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(~B), 0u));
+                            if (LaneBit(~(B << 1)))
+                                __hip_atomic_store(Mk, row_b + lane4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            const uint64_t D = r == 0 ? 0ull : (injected ? prev_U2 : (~(prev_S ^ S) & prev_U2));
+                            if (LaneBit(D))
+                                LdsMin(Mk, Rp);
+                            uint32_t root = LdsLoad(Mk);
+                            // bonds to a root other than the run's minimum link that root under it
+                            const uint64_t mg = D & __ballot(Rp != root);
+                            if (mg != 0ull)
                             {
-                                const uint64_t D = injected ? prev_U2 : (~(prev_S ^ S) & prev_U2);
-                                if (D != 0ull)
+                                uint32_t old = Rp;
+                                if (LaneBit(mg))
+                                    old = __hip_atomic_fetch_min(At(P, Rp), root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                // Rp was no longer a root (another lane linked it first): join the two trees
+                                const uint64_t more = mg & __ballot(old != Rp) & __ballot(old != root);
+                                if (more != 0ull)
                                 {
-                                    uint32_t* const M = Qw;
-                                    if (run_start == lane)
-                                        M[lane] = q0;
-                                    const bool bonded = LaneBit(D);
-                                    if (bonded)
-                                        LdsMin(&M[run_start], Rp);
-                                    const uint32_t m = LdsLoad(&M[run_start]);
-                                    root = m;
-                                    const bool merge = bonded && Rp != m;
-                                    if (__ballot(merge))
-                                    {
-                                        LdsUnionLockstep(P, merge ? Rp : 0u, merge ? m : 0u);
-                                        // labels of this row may have been linked under another root; sites
-                                        // of this row are still roots (first sites of runs without bonds up)
-                                        for (;;)
-                                        {
-                                            const uint32_t pr = root >= row_base ? root : LdsLoad(&P[root]);
-                                            if (!__ballot(pr != root))
-                                                break;
-                                            root = pr;
-                                        }
-                                    }
+                                    const bool mo = LaneBit(more);
+                                    LdsUnionLockstep(P, mo ? root : 0u, mo ? old : 0u);
                                 }
-                                if (dump)
-                                    dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
+                                // the run's minimum may itself have been linked under another root; the first
+                                // sites of runs without bonds up are roots and not in P yet
+                                for (;;)
+                                {
+                                    const uint32_t pr = root >= row_b ? root : LdsLoad(At(P, root));
+                                    if (!__ballot(pr != root))
+                                        break;
+                                    root = pr;
+                                }
                             }
-                            P[row_base + lane] = root;
+                            if (dump && r > 0)
+                                dump[prev_site] = static_cast<uint8_t>((LaneBit(prev_Ball) ? 1u : 0u) | (LaneBit(D) ? 2u : 0u));
+                            *At(Pl, row_b) = root;
                             Rp = root;
                         }
                         else
                         {
                             // every site starts under the first site of its row run
-                            P[row_base + lane] = q0;
+                            const uint64_t below = ~B & lanes_below;
+                            const uint32_t run_start = below ? 64u - static_cast<uint32_t>(__clzll(below)) : 0u;
+                            *At(Pl, row_b) = row_b + run_start * 4u;
                             if (r > 0)
                             {
                                 // bonds of the previous wave row to the row below: queue one union per overlap
@@ -569,8 +545,8 @@ namespace swmi
                                 const uint64_t K = FirstBondPerInterval(D, prev_B & tgtB, rowstart);
                                 if (LaneBit(K))
                                 {
-                                    const uint32_t top = (wr - 1u) * 64u + lane;
-                                    const uint32_t bot = (lane < 64u - tw) ? top + tw : wr * 64u + lane - (64u - tw);
+                                    const uint32_t top = row_b - 256u + lane4;
+                                    const uint32_t bot = (lane < 64u - tw) ? top + tw * 4u : row_b + lane4 - (64u - tw) * 4u;
                                     const uint32_t pos = qlen + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
                                                                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
                                     Qw[pos] = (top << 16) | bot;
@@ -607,7 +583,8 @@ namespace swmi
                     {
                         const uint32_t pos = qlen + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
                                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
-                        Qw[pos] = ((wr * 64u + lane) << 16) | (wr * 64u + lane + tw);
+                        const uint32_t top = wr * 256u + lane4;
+                        Qw[pos] = (top << 16) | (top + tw * 4u);
                     }
                     qlen += static_cast<uint32_t>(__popcll(K));
                     if (qlen >= 64u)
@@ -623,7 +600,7 @@ namespace swmi
             }
 
             // the rest of the queued unions of this tile (fewer than 64)
-            if (qlen > 0)
+            if (!TW64 && qlen > 0)
             {
                 const uint32_t e = (lane < qlen) ? Qw[lane] : 0u;
                 LdsUnionLockstep(P, e >> 16, e & 0xFFFFu);
@@ -638,20 +615,20 @@ namespace swmi
             const uint64_t Bbot = s_first_B[w + 1u];
             const uint64_t tgtB = TW64 ? Bbot : Shl(Bbot, 64u - tw);
             const bool go = LaneBit(FirstBondPerInterval(last_D, last_B & tgtB, rowstart));
-            LdsUnionLockstep(P, go ? wr * 64u + lane : 0u, go ? (wr + 1u) * 64u + lane - (64u - tw) : 0u);
+            const uint32_t top = wr * 256u + lane4;
+            LdsUnionLockstep(P, go ? top : 0u, go ? top + 256u - (64u - tw) * 4u : 0u);
         }
         __syncthreads();
 
-        // every site's block-local root (kRows chases in flight per lane); sites on the block edge mark their
-        // root. A tile of at most kRows rows keeps its roots in registers for the output below; taller
-        // tiles write them back into P.
+        // every site's block-local root (kRows chases in flight per lane). A tile of at most kRows rows
+        // keeps its roots in registers; taller tiles write them back into P.
         const bool one_chunk = nrow <= kRows;
         uint32_t xs[kRows];
         for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
         {
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
-                xs[i] = (r0 + i < nrow) ? (r_begin + r0 + i) * 64u + lane : (r_begin + r0) * 64u + lane;
+                xs[i] = (r_begin + ((r0 + i < nrow) ? r0 + i : r0)) * 256u + lane4;
             bool moving = true;
             while (__ballot(moving))
             {
@@ -659,22 +636,38 @@ namespace swmi
 #pragma unroll
                 for (uint32_t i = 0; i < kRows; ++i)
                 {
-                    const uint32_t p = LdsLoad(&P[xs[i]]);
+                    const uint32_t p = LdsLoad(At(P, xs[i]));
                     moving = moving || (p != xs[i]);
                     xs[i] = p;
                 }
             }
+            if (!one_chunk)
+            {
+#pragma unroll
+                for (uint32_t i = 0; i < kRows; ++i)
+                    if (r0 + i < nrow)
+                        *At(Pl, (r_begin + r0 + i) * 256u) = xs[i];
+            }
+        }
+        const auto root_of = [&](const uint32_t r) {
+            return one_chunk ? xs[r] : (LdsLoad(At(Pl, (r_begin + r) * 256u)) & (kPFlag - 1u));
+        };
+        __syncthreads();   // no chase may meet a flagged entry
+
+        // sites on the block edge flag their root
+        for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
+        {
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
             {
-                if (r0 + i < nrow)
+                const uint32_t r = r0 + i;
+                if (r < nrow)
                 {
-                    const uint32_t wr = r_begin + r0 + i;
-                    if (!one_chunk)
-                        P[wr * 64u + lane] = xs[i];
+                    const uint32_t wr = r_begin + r;
                     const bool on_edge = lx == 0 || rbound || (wr == 0 && lrow == 0) || (wr + 1u == bl.nwr && lrow == rpw - 1u);
                     if (on_edge)
-                        (void)__hip_atomic_fetch_or(&F[xs[i] >> 5], 1u << (xs[i] & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        (void)__hip_atomic_fetch_or(At(P, one_chunk ? xs[i] : root_of(r)), kPFlag, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }
@@ -684,23 +677,25 @@ namespace swmi
         // listed (per tile) for the resolve pass
         const uint32_t cap = d.edge_cap;
         const gptr<uint16_t> elist = Glob(d.edge_list) + (kWavesPerBlock * bl.lb + w) * cap;
+        const gptr<uint16_t> ecol_l = Glob(d.ecol) + (2u * bl.lb + (rbound ? 1u : 0u)) * d.ecol_stride + r_begin * rpw + lrow;
+        const bool col_edge = lx == 0u || rbound;
         uint32_t nedge = 0;
         const auto emit = [&](const uint32_t r, const uint32_t root) {
-            const uint32_t q = (r_begin + r) * 64u + lane;
-            const uint32_t site = bl.base + (r_begin + r) * bl.stride + lane;
-            const bool flagged = ((LdsLoad(&F[root >> 5]) >> (root & 31u)) & 1u) != 0u;
-            const uint16_t lab = static_cast<uint16_t>(root | (flagged ? kEdgeFlag : 0u));
-            local[site] = lab;
-            if (lx == 0u || rbound)
-                ecol[(2u * bl.lb + (rbound ? 1u : 0u)) * d.ecol_stride + (r_begin + r) * rpw + lrow] = lab;
-            const uint64_t K = __ballot(flagged && root == q);
+            const uint32_t wr = r_begin + r;
+            const uint32_t rowsite = Uniform(bl.base + wr * bl.stride);
+            const uint32_t p = LdsLoad(At(P, root));   // root, with kPFlag if the cluster reaches the edge
+            const uint16_t lab = static_cast<uint16_t>(p >> 2);
+            (local + rowsite)[lane] = lab;
+            if (col_edge)
+                ecol_l[r * rpw] = lab;
+            const uint64_t K = __ballot(p >= kPFlag) & __ballot(root == wr * 256u + lane4);
             if (LaneBit(K))
             {
-                parent[site] = site;
+                (parent + rowsite)[lane] = rowsite + lane;
                 const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
                                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
                 if (pos < cap)
-                    elist[pos] = static_cast<uint16_t>(q);
+                    elist[pos] = static_cast<uint16_t>(wr * 64u + lane);
             }
             nedge += static_cast<uint32_t>(__popcll(K));
         };
@@ -714,7 +709,7 @@ namespace swmi
         else
         {
             for (uint32_t r = 0; r < nrow; ++r)
-                emit(r, LdsLoad(&P[(r_begin + r) * 64u + lane]));
+                emit(r, root_of(r));
         }
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         if (lane == 0)
