@@ -19,6 +19,8 @@
 // labeller ising/src/swendsen_wang_cpu.cpp:13-205; the kernels here are written for wave64 CDNA4.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "swmi_internal.h"
 
 namespace swmi
@@ -299,12 +301,15 @@ namespace swmi
         constexpr uint32_t kCh = TWR_MAX < 16 ? TWR_MAX : 16;
         constexpr uint32_t kRows = TWR_MAX < 16 ? TWR_MAX : 16;   // rows resolved together at the end
         constexpr uint32_t kSites = kWavesPerBlock * TWR_MAX * 64;
-        constexpr uint32_t kQw = TW64 ? 64u : 128u;                // per-wave LDS words of Q
+        constexpr uint32_t kQw = 128u;                             // per-wave LDS words of Q
         constexpr uint32_t kPFlag = 1u << 17;
         static_assert(kSites * 4u <= (1u << 16), "byte offsets must fit the 16-bit halves of a queue entry");
         static_assert((kPFlag >> 2) == kEdgeFlag, "a flagged root entry shifts into a flagged label");
         __shared__ uint32_t P[kSites];
-        __shared__ uint32_t Q[kWavesPerBlock * kQw];   // 64-wide: run minima of the current row; narrow: union queue
+        // per wave: 64-wide tiles keep the run minima of the current row in words 0-63 and give each lane a
+        // word of its own in 64-127 (the target of LDS operations a lane has no use for, so that they need
+        // no exec mask); narrow tiles keep their union queue here
+        __shared__ uint32_t Q[kWavesPerBlock * kQw];
         __shared__ uint64_t s_first_B[kWavesPerBlock];
         __shared__ uint64_t AC[8];              // Numerical Recipes pairs (c << 32 | a) by selector
 
@@ -362,6 +367,7 @@ namespace swmi
         const bool last_wave = w + 1u == nwaves;
         uint32_t* const Qw = Q + w * kQw;
         uint32_t* const Pl = At(P, lane4);   // this lane's entry of wave row 0
+        uint32_t* const dummy = Qw + 64u + lane;
         // a full queue (>= 64 entries) is worked off right away; the tail moves to the front
         const auto drain64 = [&](uint32_t& qlen) {
             const uint32_t e = Qw[lane];
@@ -396,7 +402,11 @@ namespace swmi
 
             uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
             uint32_t prev_site = 0;
-            for (uint32_t r0 = 0; r0 < nrow; r0 += kCh)
+            // the chunk loop exists twice: sweeps without a group rotation (13 in 15) carry no rotation code
+            const auto chunks = [&](auto rotating) {
+            constexpr bool kRot = decltype(rotating)::value;
+            // a tile of at most kCh rows is one chunk, starting at row 0
+            for (uint32_t r0 = 0; r0 < (TWR_MAX <= kCh ? 1u : nrow); r0 += kCh)
             {
                 // The loads carry no condition (rows past the tile re-read its last row, values unused), so
                 // that they issue back to back and their latencies overlap.
@@ -411,7 +421,7 @@ namespace swmi
                 }
                 // rotation offsets of the chunk's groups, needed on rotating sweeps only
                 uint32_t g_rot = 0;
-                if (rot1 || rot2)
+                if (kRot)
                     g_rot = rotp[(bl.base + lrow_ch * bl.stride) >> 6];
                 uint32_t sp[kCh], aux[kCh], rsh[kCh];
 #pragma unroll
@@ -442,7 +452,7 @@ namespace swmi
                             // (random/src/lcg32.cpp:51-84); a rotation by `shift` moves lane
                             // (i + shift) & 63's selector to lane i
                             uint32_t ks = sp[i] & kSelMask, st = aux[i], shift = 0;
-                            if (rot1)
+                            if (kRot && rot1)
                             {
                                 shift = ShuffleShift(st, it1) & 63u;
                                 ks = static_cast<uint32_t>(__shfl(static_cast<int>(ks), static_cast<int>((lane + shift) & 63u)));
@@ -450,7 +460,7 @@ namespace swmi
                             uint64_t ac = AC[ks >> kSelShift];
                             st = static_cast<uint32_t>(ac) * st + static_cast<uint32_t>(ac >> 32);
                             const uint32_t u1 = st;
-                            if (rot2)
+                            if (kRot && rot2)
                             {
                                 const uint32_t s2 = ShuffleShift(st, it2) & 63u;
                                 shift += s2;
@@ -459,7 +469,7 @@ namespace swmi
                             }
                             st = static_cast<uint32_t>(ac) * st + static_cast<uint32_t>(ac >> 32);
                             (rng + Uniform(site - lane))[lane] = st;
-                            if (rot1 || rot2)
+                            if (kRot)
                             {
                                 spins[site] = static_cast<uint8_t>((sp[i] & 1u) | ks);
                                 if (lane == 0)
@@ -494,31 +504,31 @@ namespace swmi
                             // lane; the first lane of the run initialises it with its own node
                             uint32_t* const Mk = Qw + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(~B >> 32),
                                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(~B), 0u));
-                            if (LaneBit(~(B << 1)))
-                                __hip_atomic_store(Mk, row_b + lane4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            const uint32_t self = row_b + lane4;
+                            __hip_atomic_store(LaneBit(~(B << 1)) ? Mk : dummy, self, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            // the row's entries start as roots (overwritten below), so chases end on them
+                            __hip_atomic_store(At(P, self), self, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             const uint64_t D = r == 0 ? 0ull : (injected ? prev_U2 : (~(prev_S ^ S) & prev_U2));
-                            if (LaneBit(D))
-                                LdsMin(Mk, Rp);
+                            if (r > 0)
+                                LdsMin(LaneBit(D) ? Mk : dummy, Rp);
                             uint32_t root = LdsLoad(Mk);
                             // bonds to a root other than the run's minimum link that root under it
                             const uint64_t mg = D & __ballot(Rp != root);
                             if (mg != 0ull)
                             {
-                                uint32_t old = Rp;
-                                if (LaneBit(mg))
-                                    old = __hip_atomic_fetch_min(At(P, Rp), root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                const uint32_t old = __hip_atomic_fetch_min(LaneBit(mg) ? At(P, Rp) : dummy, root, __ATOMIC_RELAXED,
+                                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                                 // Rp was no longer a root (another lane linked it first): join the two trees
-                                const uint64_t more = mg & __ballot(old != Rp) & __ballot(old != root);
+                                const uint64_t more = mg & __ballot(old != Rp && old != root);
                                 if (more != 0ull)
                                 {
                                     const bool mo = LaneBit(more);
                                     LdsUnionLockstep(P, mo ? root : 0u, mo ? old : 0u);
                                 }
-                                // the run's minimum may itself have been linked under another root; the first
-                                // sites of runs without bonds up are roots and not in P yet
+                                // the run's minimum may itself have been linked under another root
                                 for (;;)
                                 {
-                                    const uint32_t pr = root >= row_b ? root : LdsLoad(At(P, root));
+                                    const uint32_t pr = LdsLoad(At(P, root));
                                     if (!__ballot(pr != root))
                                         break;
                                     root = pr;
@@ -566,6 +576,11 @@ namespace swmi
                     }
                 }
             }
+            };
+            if (rot1 || rot2)
+                chunks(std::true_type{});
+            else
+                chunks(std::false_type{});
 
             // bonds out of the last wave row: into the same wave row (tw < 64), into the next tile of the
             // block, or across the block edge
@@ -654,22 +669,26 @@ namespace swmi
         };
         __syncthreads();   // no chase may meet a flagged entry
 
-        // sites on the block edge flag their root
-        for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
+        // sites on the block edge flag their root (lanes off the edge flag their own word of Q)
+        const uint64_t side = TW64 ? 0x8000000000000001ull : __ballot(lx == 0 || rbound);
+        const uint64_t top_lanes = TW64 ? ~0ull : __ballot(lrow == 0);
+        const uint64_t bottom_lanes = TW64 ? ~0ull : __ballot(lrow == rpw - 1u);
+        const auto mark = [&](const uint32_t r, const uint32_t root) {
+            const uint32_t wr = r_begin + r;
+            const uint64_t m = side | (wr == 0 ? top_lanes : 0ull) | (wr + 1u == bl.nwr ? bottom_lanes : 0ull);
+            (void)__hip_atomic_fetch_or(LaneBit(m) ? At(P, root) : dummy, kPFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        if (one_chunk)
         {
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
-            {
-                const uint32_t r = r0 + i;
-                if (r < nrow)
-                {
-                    const uint32_t wr = r_begin + r;
-                    const bool on_edge = lx == 0 || rbound || (wr == 0 && lrow == 0) || (wr + 1u == bl.nwr && lrow == rpw - 1u);
-                    if (on_edge)
-                        (void)__hip_atomic_fetch_or(At(P, one_chunk ? xs[i] : root_of(r)), kPFlag, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
-            }
+                if (i < nrow)
+                    mark(i, xs[i]);
+        }
+        else
+        {
+            for (uint32_t r = 0; r < nrow; ++r)
+                mark(r, root_of(r));
         }
         __syncthreads();
 
@@ -678,24 +697,23 @@ namespace swmi
         const uint32_t cap = d.edge_cap;
         const gptr<uint16_t> elist = Glob(d.edge_list) + (kWavesPerBlock * bl.lb + w) * cap;
         const gptr<uint16_t> ecol_l = Glob(d.ecol) + (2u * bl.lb + (rbound ? 1u : 0u)) * d.ecol_stride + r_begin * rpw + lrow;
-        const bool col_edge = lx == 0u || rbound;
         uint32_t nedge = 0;
+        // A tile lists at most min(tile sites, block boundary sites) roots, which is the list's capacity.
         const auto emit = [&](const uint32_t r, const uint32_t root) {
             const uint32_t wr = r_begin + r;
             const uint32_t rowsite = Uniform(bl.base + wr * bl.stride);
             const uint32_t p = LdsLoad(At(P, root));   // root, with kPFlag if the cluster reaches the edge
             const uint16_t lab = static_cast<uint16_t>(p >> 2);
             (local + rowsite)[lane] = lab;
-            if (col_edge)
+            if (LaneBit(side))
                 ecol_l[r * rpw] = lab;
-            const uint64_t K = __ballot(p >= kPFlag) & __ballot(root == wr * 256u + lane4);
+            const uint64_t K = __ballot(p >= kPFlag && root == wr * 256u + lane4);
             if (LaneBit(K))
             {
                 (parent + rowsite)[lane] = rowsite + lane;
                 const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
                                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
-                if (pos < cap)
-                    elist[pos] = static_cast<uint16_t>(wr * 64u + lane);
+                elist[pos] = static_cast<uint16_t>(wr * 64u + lane);
             }
             nedge += static_cast<uint32_t>(__popcll(K));
         };
