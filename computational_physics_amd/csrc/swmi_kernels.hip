@@ -43,6 +43,16 @@ namespace swmi
     using u32x2 = uint32_t __attribute__((ext_vector_type(2)));
     using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
 
+    // Buffer views (raw: stride 0, byte offsets). The hardware checks a lane's offset against the view's
+    // size and drops a store past it, which lets a lane opt out of a store without an exec mask.
+    using Rsrc = __amdgpu_buffer_rsrc_t;
+    constexpr uint32_t kNoStore = 0x80000000u;   // a lane offset past every view
+    template <typename T>
+    __device__ __forceinline__ Rsrc MakeRsrc(T* base, const uint32_t bytes)
+    {
+        return __builtin_amdgcn_make_buffer_rsrc(static_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+    }
+
     // bit `lane` of a wave-uniform mask, as a per-lane predicate (v_cndmask on the SGPR pair)
     __device__ __forceinline__ bool LaneBit(const uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
 
@@ -358,8 +368,6 @@ namespace swmi
         const gptr<uint32_t> rng = Glob(d.rng);
         const gptr<uint32_t> rotp = Glob(d.rot);
         const gptr<const uint8_t> bond_in = Glob(d.bond_in);
-        const gptr<uint16_t> local = Glob(d.local);
-        const gptr<uint32_t> parent = Glob(d.parent);
 
         const uint32_t r_begin = w * twr;
         const uint32_t nrow = bl.nwr > r_begin ? min(twr, bl.nwr - r_begin) : 0u;
@@ -637,24 +645,45 @@ namespace swmi
 
         // every site's block-local root (kRows chases in flight per lane). A tile of at most kRows rows
         // keeps its roots in registers; taller tiles write them back into P.
-        const bool one_chunk = nrow <= kRows;
+        const bool one_chunk = TWR_MAX <= kRows || nrow <= kRows;
         uint32_t xs[kRows];
-        for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
+        // (a wave of a partial block may own no rows: its part of P was never written and is not chased)
+        for (uint32_t r0 = 0; r0 < (TWR_MAX <= kRows ? (nrow > 0 ? 1u : 0u) : nrow); r0 += kRows)
         {
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
                 xs[i] = (r_begin + ((r0 + i < nrow) ? r0 + i : r0)) * 256u + lane4;
-            bool moving = true;
-            while (__ballot(moving))
+            // two steps per round; a round ends the chase once a step moved no lane of any row. The
+            // differences are summed in vector registers (16 values below 2^17 cannot wrap to zero). Parents
+            // only decrease, so a chase ends within kSites steps; more rounds mean a broken forest, which is
+            // reported instead of followed.
+            for (uint32_t round = 0;; ++round)
             {
-                moving = false;
+                if (round * 2u > kSites)
+                {
+                    if (lane == 0)
+                        (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                uint32_t ys[kRows];
+                uint32_t moved = 0;
 #pragma unroll
                 for (uint32_t i = 0; i < kRows; ++i)
                 {
-                    const uint32_t p = LdsLoad(At(P, xs[i]));
-                    moving = moving || (p != xs[i]);
-                    xs[i] = p;
+                    ys[i] = LdsLoad(At(P, xs[i]));
+                    moved += ys[i] ^ xs[i];
                 }
+                if (!__ballot(moved != 0u))
+                    break;
+                moved = 0;
+#pragma unroll
+                for (uint32_t i = 0; i < kRows; ++i)
+                {
+                    xs[i] = LdsLoad(At(P, ys[i]));
+                    moved += xs[i] ^ ys[i];
+                }
+                if (!__ballot(moved != 0u))
+                    break;
             }
             if (!one_chunk)
             {
@@ -669,61 +698,81 @@ namespace swmi
         };
         __syncthreads();   // no chase may meet a flagged entry
 
-        // sites on the block edge flag their root (lanes off the edge flag their own word of Q)
+        // sites on the block edge flag their root (lanes off the edge flag their own word of Q). The edge
+        // lanes of a row are `side`, plus every lane of the block's first and last lattice rows.
         const uint64_t side = TW64 ? 0x8000000000000001ull : __ballot(lx == 0 || rbound);
-        const uint64_t top_lanes = TW64 ? ~0ull : __ballot(lrow == 0);
-        const uint64_t bottom_lanes = TW64 ? ~0ull : __ballot(lrow == rpw - 1u);
-        const auto mark = [&](const uint32_t r, const uint32_t root) {
-            const uint32_t wr = r_begin + r;
-            const uint64_t m = side | (wr == 0 ? top_lanes : 0ull) | (wr + 1u == bl.nwr ? bottom_lanes : 0ull);
-            (void)__hip_atomic_fetch_or(LaneBit(m) ? At(P, root) : dummy, kPFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        };
-        if (one_chunk)
-        {
+        const uint64_t top_first = r_begin == 0 ? (TW64 ? ~0ull : __ballot(lrow == 0)) : 0ull;
+        const uint64_t bottom_last = last_wave ? (TW64 ? ~0ull : __ballot(lrow == rpw - 1u)) : 0ull;
+        // full tiles (the rule) take the row count and the first/last row masks at compile time
+        const bool full = TWR_MAX <= kRows && nrow == kRows;
+        const auto mark_rows = [&](auto full_t) {
+            constexpr bool kFull = decltype(full_t)::value;
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
-                if (i < nrow)
-                    mark(i, xs[i]);
-        }
+            {
+                if (kFull || i < nrow)
+                {
+                    const bool last = kFull ? i + 1u == kRows : i + 1u == nrow;
+                    const uint64_t m = side | (i == 0 ? top_first : 0ull) | (last ? bottom_last : 0ull);
+                    (void)__hip_atomic_fetch_or(LaneBit(m) ? At(P, xs[i]) : dummy, kPFlag, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        };
+        if (full)
+            mark_rows(std::true_type{});
+        else if (one_chunk)
+            mark_rows(std::false_type{});
         else
         {
             for (uint32_t r = 0; r < nrow; ++r)
-                mark(r, root_of(r));
+            {
+                const uint64_t m = side | (r == 0 ? top_first : 0ull) | (r + 1u == nrow ? bottom_last : 0ull);
+                (void)__hip_atomic_fetch_or(LaneBit(m) ? At(P, root_of(r)) : dummy, kPFlag, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
         __syncthreads();
 
         // labels out; the roots of edge-reaching clusters start their global union-find entry and are
-        // listed (per tile) for the resolve pass
+        // listed (per tile) for the resolve pass. The outputs are written through buffer views of this
+        // block's part of each array; a lane with nothing to store gets an offset past the view, and the
+        // hardware drops that store (no exec mask). A tile lists at most min(tile sites, block boundary
+        // sites) roots, which is the list's capacity.
         const uint32_t cap = d.edge_cap;
-        const gptr<uint16_t> elist = Glob(d.edge_list) + (kWavesPerBlock * bl.lb + w) * cap;
-        const gptr<uint16_t> ecol_l = Glob(d.ecol) + (2u * bl.lb + (rbound ? 1u : 0u)) * d.ecol_stride + r_begin * rpw + lrow;
+        const uint32_t span = (bl.nwr - 1u) * bl.stride + 64u;   // sites from the block's first to its last
+        const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
+        const Rsrc rs_parent = MakeRsrc(d.parent + bl.base, span * 4u);
+        const Rsrc rs_elist = MakeRsrc(d.edge_list + (kWavesPerBlock * bl.lb + w) * cap, cap * 2u);
+        const Rsrc rs_ecol = MakeRsrc(d.ecol + 2u * bl.lb * d.ecol_stride, 4u * d.ecol_stride);
+        const uint32_t ecol_off = LaneBit(side) ? ((rbound ? d.ecol_stride : 0u) + r_begin * rpw + lrow) * 2u : kNoStore;
         uint32_t nedge = 0;
-        // A tile lists at most min(tile sites, block boundary sites) roots, which is the list's capacity.
         const auto emit = [&](const uint32_t r, const uint32_t root) {
             const uint32_t wr = r_begin + r;
-            const uint32_t rowsite = Uniform(bl.base + wr * bl.stride);
+            const uint32_t rowoff = wr * bl.stride;   // sites from the block's first site
             const uint32_t p = LdsLoad(At(P, root));   // root, with kPFlag if the cluster reaches the edge
-            const uint16_t lab = static_cast<uint16_t>(p >> 2);
-            (local + rowsite)[lane] = lab;
-            if (LaneBit(side))
-                ecol_l[r * rpw] = lab;
+            const uint32_t lab = p >> 2;
+            __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(lab), rs_local, lane * 2u, rowoff * 2u, 0);
+            __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(lab), rs_ecol, ecol_off, r * rpw * 2u, 0);
             const uint64_t K = __ballot(p >= kPFlag && root == wr * 256u + lane4);
-            if (LaneBit(K))
-            {
-                (parent + rowsite)[lane] = rowsite + lane;
-                const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
-                elist[pos] = static_cast<uint16_t>(wr * 64u + lane);
-            }
+            const bool listed = LaneBit(K);
+            __builtin_amdgcn_raw_buffer_store_b32(bl.base + rowoff + lane, rs_parent, listed ? lane4 : kNoStore, rowoff * 4u, 0);
+            const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
+            __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(wr * 64u + lane), rs_elist, pos * 2u + (listed ? 0u : kNoStore), 0, 0);
             nedge += static_cast<uint32_t>(__popcll(K));
         };
-        if (one_chunk)
-        {
+        const auto emit_rows = [&](auto full_t) {
+            constexpr bool kFull = decltype(full_t)::value;
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
-                if (i < nrow)
+                if (kFull || i < nrow)
                     emit(i, xs[i]);
-        }
+        };
+        if (full)
+            emit_rows(std::true_type{});
+        else if (one_chunk)
+            emit_rows(std::false_type{});
         else
         {
             for (uint32_t r = 0; r < nrow; ++r)

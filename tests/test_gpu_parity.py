@@ -103,6 +103,24 @@ def test_sweeps_bit_exact(swmi, oracle, n0, n1, T, tile_rows):
         assert e.error_flags() == 0
 
 
+@pytest.mark.parametrize("n0,n1,tile_rows", [(64, 32, 16), (64, 16, 8), (32, 32, 8), (32, 32, 16), (64, 64, 32),
+                                             (32, 64, 32)])
+def test_blocks_with_empty_tiles_bit_exact(swmi, oracle, n0, n1, tile_rows):
+    """A lattice shorter than one block (4 tiles of tile_rows wave rows) leaves the block's last tiles without
+    rows, in every kernel class (64-wide and narrow tiles, heights 8, 16, 32); their wavefronts take no part in
+    the labelling."""
+    print(f"empty tiles {n0}x{n1} tile_rows {tile_rows}", flush=True)
+    ref = oracle.PerSite(n0, n1, TC, seed=13, spin_seed=7)
+    with _engine(swmi, n0, n1, seed=13, spin_seed=7, tile_rows=tile_rows) as e:
+        assert e.tile_rows() == tile_rows
+        for k in range(1, 21):
+            e.sweep(1)
+            ref.sweep()
+            if k % 5 == 0:
+                np.testing.assert_array_equal(e.spins(), ref.spins, err_msg=f"sweep {k}")
+        assert e.error_flags() == 0
+
+
 def test_batch_equals_alone(swmi):
     cfgs = [swmi.LatticeConfig(64, 64, 2.0, seed=1, spin_seed=2),
             swmi.LatticeConfig(32, 32, 2.9, seed=2, spin_seed=3, tile_rows=4),
