@@ -12,9 +12,10 @@
 // per-group data except on the sweeps that rotate (one in 7.5).
 //   k_merge       per block: unions across block edges by atomicMin on the uint32 parent array in HBM,
 //                 over the roots of edge-reaching clusters only.
-//   k_flip        per block: the listed edge-cluster roots are chased in the global parent array (one
-//                 chase per root, kept in LDS); then every site's label is its block-local root or that
-//                 LDS entry, and spin ^= label & 1. Advances the lcg32 iteration count.
+//   k_flip        per block (two waves): the listed edge-cluster roots are chased in the global parent
+//                 array (one chase per root) and their flip bits kept in an LDS bitmap; then every site
+//                 flips by its edge root's bit or by its block-local root's parity (spin ^= label & 1).
+//                 Advances the lcg32 iteration count.
 // Reference behaviour: ising/src/swendsen_wang_amd_gpu.cpp:78-467 (four kernels per sweep) and the CPU
 // labeller ising/src/swendsen_wang_cpu.cpp:13-205; the kernels here are written for wave64 CDNA4.
 #include <hip/hip_runtime.h>
@@ -904,14 +905,146 @@ namespace swmi
     // k_flip: resolve + flip (reference ResolveLabels_Kernel / FlipClusters_Kernel,
     // swendsen_wang_amd_gpu.cpp:403-467)
     // =============================================================================================
-    // Two phases per block. (1) The roots of the block's edge-reaching clusters (the tile lists written
-    // by k_bond_label) are chased in the global parent array, one root per thread; the canonical label
-    // found is kept in LDS at the root's block-local index. (2) The block is streamed four sites per
-    // lane: a site's label is its block-local root mapped to a site index or, for an edge cluster, the
-    // LDS entry of that root; the spin flips by the label's parity. The first chunk of phase 2 is
-    // loaded before phase 1 so that its memory latency overlaps the chases.
-    __global__ __launch_bounds__(256) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
-                                                  const uint32_t total_blocks, const uint32_t flags)
+    // Sweep kernel. Two waves per block, so that every block of a 4096^2 lattice is resident at once and
+    // the latency of phase 1 is paid once per launch. (1) The roots of the block's edge-reaching clusters
+    // (the tile lists written by k_bond_label) are chased in the global parent array, one root per
+    // thread; the flip bit of the canonical label found goes into an LDS bitmap at the root's block-local
+    // index. (2) The block is streamed four sites per lane: a site flips by that bit for an edge cluster,
+    // or by the bit of its block-local root's site index otherwise. The first chunk of phase 2 is loaded
+    // before phase 1 so that its memory latency overlaps the chases.
+    __device__ __forceinline__ uint32_t FlipBit(const uint32_t root, const bool random_rule, const uint32_t key)
+    {
+        return random_rule ? (Fmix32(root ^ key) >> 31) : (root & 1u);
+    }
+
+    __global__ __launch_bounds__(128) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                  const uint32_t total_blocks)
+    {
+        constexpr uint32_t kThreads = 128;
+        constexpr uint32_t kPass = kThreads / 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
+        constexpr uint32_t kCh = 8;                 // passes loaded together
+        constexpr uint32_t kMaxSites = kWavesPerBlock * 64u * 64u;
+        __shared__ uint32_t fbits[kMaxSites / 32];
+        const uint32_t b = RemapBlock(total_blocks);
+        if (b >= total_blocks)
+            return;
+        const LatDesc& d = descs[block_lat[b]];
+        const gptr<uint32_t> stp = Glob(d.state);
+        if (stp[kStActive] == 0u)
+            return;
+        const Block bl = DecodeBlock(d, b);
+        const gptr<uint32_t> L = Glob(d.parent);
+        const gptr<uint16_t> local = Glob(d.local);
+        const gptr<uint8_t> spins = Glob(d.spins);
+        const uint32_t qmax = bl.nwr * 64u;
+        const bool random_rule = d.flip_rule != 0u;
+        const uint32_t key = random_rule ? stp[kStFlipKey] : 0u;
+        const uint32_t sub = threadIdx.x >> 4;
+        const uint32_t col = (threadIdx.x & 15u) * 4u;
+        bool ok = true;
+
+        u32x2 lv[kCh];
+        uint32_t sv[kCh];
+        const auto load = [&](const uint32_t r0) {
+#pragma unroll
+            for (uint32_t k = 0; k < kCh; ++k)
+            {
+                const uint32_t r = r0 + sub + k * kPass;
+                lv[k] = u32x2{0u, 0u};
+                sv[k] = 0u;
+                if (r < bl.nwr)
+                {
+                    const uint32_t site = bl.base + r * bl.stride + col;
+                    lv[k] = *(gptr<const u32x2>)(local + site);
+                    sv[k] = *(gptr<const uint32_t>)(spins + site);
+                }
+            }
+        };
+        load(0u);
+        for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
+            fbits[i] = 0u;
+        __syncthreads();
+
+        // phase 1: flip bits of the canonical labels of the edge-reaching roots
+        {
+            const uint32_t cap = d.edge_cap;
+            const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
+            const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
+            const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
+            const gptr<uint16_t> elist = Glob(d.edge_list) + kWavesPerBlock * bl.lb * cap;
+            for (uint32_t i = threadIdx.x; i < total; i += kThreads)
+            {
+                const uint32_t j = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
+                const uint32_t first = j == 0u ? 0u : (j == 1u ? e1 : (j == 2u ? e2 : e3));
+                const uint32_t q = elist[j * cap + (i - first)];
+                if (q >= qmax)
+                {
+                    ok = false;
+                    continue;
+                }
+                uint32_t x = SiteOf(bl.base, bl.stride, q);
+                uint32_t p = CachedLoad(&L[x]);
+                while (p != x)
+                {
+                    if (p > x)
+                    {
+                        ok = false;
+                        break;
+                    }
+                    x = p;
+                    p = CachedLoad(&L[x]);
+                }
+                if (FlipBit(x, random_rule, key))
+                    (void)__hip_atomic_fetch_or(&fbits[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        __syncthreads();
+
+        // phase 2: flips, four sites per lane. Block bases and row strides are even, so the parity of a
+        // block-local root index is the parity of its site index.
+        for (uint32_t r0 = 0;;)
+        {
+#pragma unroll
+            for (uint32_t k = 0; k < kCh; ++k)
+            {
+                const uint32_t r = r0 + sub + k * kPass;
+                if (r < bl.nwr)
+                {
+                    const uint32_t site = bl.base + r * bl.stride + col;
+                    const uint32_t lab[4] = {lv[k].x & 0xFFFFu, lv[k].x >> 16, lv[k].y & 0xFFFFu, lv[k].y >> 16};
+                    uint32_t flip = 0;
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e)
+                    {
+                        uint32_t q = lab[e] & 0x7FFFu;
+                        const bool edge = (lab[e] & kEdgeFlag) != 0u;
+                        if (q >= qmax)
+                        {
+                            ok = false;
+                            q = 0;
+                        }
+                        const uint32_t own = random_rule ? FlipBit(SiteOf(bl.base, bl.stride, q), true, key) : (q & 1u);
+                        const uint32_t bit = edge ? ((LdsLoad(&fbits[q >> 5]) >> (q & 31u)) & 1u) : own;
+                        flip |= bit << (8u * e);
+                    }
+                    *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;
+                }
+            }
+            r0 += kCh * kPass;
+            if (r0 >= bl.nwr)
+                break;
+            load(r0);
+        }
+        if (!ok)
+            (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bl.lb == 0 && threadIdx.x == 0)
+            stp[kStIteration] += 2u;
+    }
+
+    // Inspection kernel (labels of every site, no flips): the same two phases with the canonical label
+    // itself kept in LDS (one word per block site) and written to parent[] for every site.
+    __global__ __launch_bounds__(256) void k_write_labels(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                          const uint32_t total_blocks, const uint32_t flags)
     {
         constexpr uint32_t kPass = 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
         constexpr uint32_t kCh = 4;      // passes loaded together
@@ -1310,8 +1443,13 @@ namespace swmi
     void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
                     const uint32_t flags, hipStream_t stream)
     {
+        if (flags == (kFlip | kAdvance))
+        {
+            k_flip<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            return;
+        }
         const size_t lds = static_cast<size_t>(kWavesPerBlock) * max_twr * 64u * sizeof(uint32_t);
-        k_flip<<<XcdGrid(total_blocks), 256, lds, stream>>>(d_desc, d_block_lat, total_blocks, flags);
+        k_write_labels<<<XcdGrid(total_blocks), 256, lds, stream>>>(d_desc, d_block_lat, total_blocks, flags);
     }
 
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t sample_mode,
