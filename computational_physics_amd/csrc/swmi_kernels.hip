@@ -366,12 +366,16 @@ namespace swmi
         const bool rot2 = !injected && (it2 % 15u == 0u);
         const gptr<uint8_t> dump = DBG ? Glob(d.bond_dump) : nullptr;
         const gptr<uint8_t> spins = Glob(d.spins);
-        const gptr<uint32_t> rng = Glob(d.rng);
         const gptr<uint32_t> rotp = Glob(d.rot);
         const gptr<const uint8_t> bond_in = Glob(d.bond_in);
 
         const uint32_t r_begin = w * twr;
         const uint32_t nrow = bl.nwr > r_begin ? min(twr, bl.nwr - r_begin) : 0u;
+        // buffer views of this block's part of the per-site arrays (row offsets in scalar registers, the lane
+        // offset in a vector register, no per-row vector addresses)
+        const uint32_t span = (bl.nwr - 1u) * bl.stride + 64u;   // sites from the block's first to its last
+        const Rsrc rs_spins = MakeRsrc(d.spins + bl.base, span);
+        const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
         const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
         const bool last_wave = w + 1u == nwaves;
         uint32_t* const Qw = Q + w * kQw;
@@ -438,9 +442,9 @@ namespace swmi
                 {
                     // row base in scalar registers, the lane offset in a vector register
                     const uint32_t wr = r_begin + min(r0 + i, nrow - 1u);
-                    const uint32_t rowbase = Uniform(bl.base + wr * bl.stride);
-                    sp[i] = (spins + rowbase)[lane];
-                    aux[i] = injected ? (bond_in + rowbase)[lane] : (rng + rowbase)[lane];
+                    const uint32_t rowoff = Uniform(wr * bl.stride);
+                    sp[i] = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, lane, rowoff, 0);
+                    aux[i] = injected ? (bond_in + bl.base + rowoff)[lane] : __builtin_amdgcn_raw_buffer_load_b32(rs_rng, lane4, rowoff * 4u, 0);
                     rsh[i] = TW64 ? 0u : spins[((bl.wyb + wr) * rpw + lrow) * n0 + x_right];
                 }
                 const uint64_t HR = TW64 ? __ballot((h & 1u) != 0u) : 0ull;
@@ -477,7 +481,7 @@ namespace swmi
                                 ac = AC[ks >> kSelShift];
                             }
                             st = static_cast<uint32_t>(ac) * st + static_cast<uint32_t>(ac >> 32);
-                            (rng + Uniform(site - lane))[lane] = st;
+                            __builtin_amdgcn_raw_buffer_store_b32(st, rs_rng, lane4, Uniform(wr * bl.stride) * 4u, 0);
                             if (kRot)
                             {
                                 spins[site] = static_cast<uint8_t>((sp[i] & 1u) | ks);
@@ -741,7 +745,6 @@ namespace swmi
         // hardware drops that store (no exec mask). A tile lists at most min(tile sites, block boundary
         // sites) roots, which is the list's capacity.
         const uint32_t cap = d.edge_cap;
-        const uint32_t span = (bl.nwr - 1u) * bl.stride + 64u;   // sites from the block's first to its last
         const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
         const Rsrc rs_parent = MakeRsrc(d.parent + bl.base, span * 4u);
         const Rsrc rs_elist = MakeRsrc(d.edge_list + (kWavesPerBlock * bl.lb + w) * cap, cap * 2u);
