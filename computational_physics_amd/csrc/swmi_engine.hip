@@ -129,6 +129,7 @@ namespace swmi
         // distinct clusters reaching a block's boundary <= boundary sites (2 columns + 2 rows)
         g.edge_cap = std::min<uint32_t>(g.twr * 64u, 2u * g.tw + 2u * kWavesPerBlock * g.twr * g.rpw);
         g.ecol_stride = kWavesPerBlock * g.twr * g.rpw;
+        g.halo_stride = g.ecol_stride + 64u;
         return g;
     }
 
@@ -138,7 +139,8 @@ namespace swmi
         const uint64_t n = static_cast<uint64_t>(c.n0) * c.n1, groups = n / 64;
         return n * (1 + 4 + 2 + 4) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
                4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
-               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4) + 4ull * g.ecol_stride * g.n_blocks;
+               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4) + 4ull * g.ecol_stride * g.n_blocks +
+               32ull * g.twr * g.n_blocks + static_cast<uint64_t>(g.halo_stride) * g.n_blocks;
     }
 
     bool Lattice::SameShape(const swmi_lattice_config& c) const
@@ -150,8 +152,8 @@ namespace swmi
     uint64_t Lattice::Bytes() const
     {
         return spins.bytes() + rng.bytes() + masks.bytes() + rot.bytes() + local.bytes() + parent.bytes() + bbits.bytes() +
-               state.bytes() + partials.bytes() + edge_list.bytes() + edge_count.bytes() + ecol.bytes() + bond_dump.bytes() +
-               bond_in.bytes();
+               state.bytes() + partials.bytes() + edge_list.bytes() + edge_count.bytes() + ecol.bytes() + fbits.bytes() +
+               halo.bytes() + bond_dump.bytes() + bond_in.bytes();
     }
 
     void Lattice::Allocate(const swmi_lattice_config& c)
@@ -173,6 +175,8 @@ namespace swmi
         edge_list.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
         edge_count.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
         ecol.allocate(2ull * g.ecol_stride * g.n_blocks);
+        fbits.allocate(8ull * g.twr * g.n_blocks);
+        halo.allocate(static_cast<size_t>(g.halo_stride) * g.n_blocks);
     }
 
     void Lattice::Initialize(const swmi_lattice_config& c, hipStream_t stream)
@@ -286,6 +290,9 @@ LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
     d.n_blocks = l.g.n_blocks;
     d.flip_rule = l.cfg.flip_rule;
     d.seed = l.cfg.seed;
+    d.fbits = l.fbits.ptr;
+    d.halo = l.halo.ptr;
+    d.halo_stride = l.g.halo_stride;
     return d;
 }
 
@@ -372,7 +379,7 @@ void swmi_engine::EnqueueSweeps(uint32_t n)
         {
             LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, recording, s);
             LaunchMerge(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
-            LaunchFlip(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, kFlip | kAdvance, s);
+            LaunchResolve(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
         }
     }
     for (size_t c = 1; c < nc; ++c)
@@ -383,10 +390,18 @@ void swmi_engine::EnqueueSweeps(uint32_t n)
     LaunchCheck("sweep launch");
 }
 
+void swmi_engine::EnqueueMaterialize()
+{
+    for (auto& k : classes)
+        LaunchMaterialize(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, stream);
+    LaunchCheck("flip launch");
+}
+
 void swmi_engine::EnqueueSample(uint32_t mode)
 {
     if (descs.empty())
         return;
+    EnqueueMaterialize();
     for (auto& k : classes)
         LaunchObservables(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, mode, stream);
     LaunchFinalizeSamples(d_desc.ptr, static_cast<uint32_t>(descs.size()), mode, stream);
@@ -482,6 +497,13 @@ void swmi_engine::OneLattice(const Lattice& l, const uint8_t* bond_in)
     }
     Check(hipMemcpyAsync(d_one_desc.ptr, &d, sizeof(d), hipMemcpyHostToDevice, stream), "descriptor upload");
     Check(hipStreamSynchronize(stream), "descriptor upload");
+}
+
+void swmi_engine::MaterializeOne(const Lattice& l)
+{
+    OneLattice(l, nullptr);
+    LaunchMaterialize(d_one_desc.ptr, d_zero_blocks.ptr, l.g.n_blocks, stream);
+    LaunchCheck("flip launch");
 }
 
 namespace
@@ -598,6 +620,7 @@ int swmi_upload_spins(swmi_engine* engine, uint32_t lattice, const uint8_t* spin
         tmp.allocate(l.n);
         Check(hipMemcpyAsync(tmp.ptr, spins, l.n, hipMemcpyHostToDevice, engine->stream), "spin upload");
         LaunchSetSpins(l.spins.ptr, tmp.ptr, l.n, engine->stream);
+        LaunchSetApplied(l.state.ptr, engine->stream);   // the uploaded spins are the chain's state
         engine->LaunchCheck("spin upload");
         Check(hipStreamSynchronize(engine->stream), "spin upload");
     });
@@ -609,6 +632,7 @@ int swmi_download_spins(swmi_engine* engine, uint32_t lattice, uint8_t* spins)
         if (!spins)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = Eng(engine)->At(lattice);
+        engine->MaterializeOne(l);
         Check(hipMemcpyAsync(spins, l.spins.ptr, l.n, hipMemcpyDeviceToHost, engine->stream), "spin download");
         Check(hipStreamSynchronize(engine->stream), "spin download");
         for (uint64_t i = 0; i < l.n; ++i)   // bits 3-5 carry the lcg32 selectors
@@ -710,7 +734,7 @@ int swmi_observables(swmi_engine* engine, uint32_t lattice, int64_t* energy, int
 {
     return Guard([&] {
         Lattice& l = Eng(engine)->At(lattice);
-        engine->OneLattice(l, nullptr);
+        engine->MaterializeOne(l);
         LaunchObservables(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, kSampleAlways, engine->stream);
         engine->LaunchCheck("observables launch");
         std::vector<int32_t> p(2ull * kWavesPerBlock * l.g.n_blocks);
@@ -838,10 +862,11 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
         if (!l.bond_in.ptr)
             l.bond_in.allocate(l.n);
         Check(hipMemcpyAsync(l.bond_in.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
+        engine->MaterializeOne(l);   // the injected pass replaces the labels a pending flip needs
         engine->OneLattice(l, l.bond_in.ptr);
         LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, true, engine->stream);
         LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, engine->stream);
-        LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, kWriteLabels, engine->stream);
+        LaunchWriteLabels(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("label launch");
         Check(hipStreamSynchronize(engine->stream), "label");
     });
@@ -854,7 +879,7 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = Eng(engine)->At(lattice);
         engine->OneLattice(l, nullptr);
-        LaunchFlip(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, kWriteLabels, engine->stream);
+        LaunchWriteLabels(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("resolve launch");
         Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
         Check(hipStreamSynchronize(engine->stream), "label download");
@@ -913,6 +938,7 @@ int swmi_checkpoint(swmi_engine* engine, uint32_t lattice, void* buffer, uint64_
         Lattice& l = Eng(engine)->At(lattice);
         if (!buffer || bytes < CheckpointBytes(l))
             throw SwmiError(SWMI_ERR_INVALID, "checkpoint buffer too small (" + std::to_string(CheckpointBytes(l)) + " bytes needed)");
+        engine->MaterializeOne(l);   // the checkpoint holds the chain's spins, with no flip pending
         CheckpointHeader h{};
         std::memcpy(h.magic, "SWMICKP1", 8);
         h.n0 = l.cfg.n0;
@@ -971,6 +997,7 @@ int swmi_restore(swmi_engine* engine, uint32_t lattice, const void* buffer, uint
         put(l.masks.ptr, 24ull * l.n_groups);
         put(l.rot.ptr, 4ull * l.n_groups);
         put(l.state.ptr, 4ull * l.state_words);
+        LaunchSetApplied(l.state.ptr, engine->stream);   // a checkpoint's spins hold the chain's state
         Check(hipStreamSynchronize(engine->stream), "restore");
         l.cfg.seed = c.seed;
         l.cfg.flip_rule = c.flip_rule;
@@ -1078,7 +1105,7 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
                 LaunchMerge(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, engine->stream);
             Check(hipEventRecord(ev[2], engine->stream), "event");
             for (auto& k : engine->classes)
-                LaunchFlip(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, kFlip | kAdvance, engine->stream);
+                LaunchResolve(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, engine->stream);
             Check(hipEventRecord(ev[3], engine->stream), "event");
             Check(hipEventSynchronize(ev[3]), "event sync");
             for (int k = 0; k < 3; ++k)

@@ -108,7 +108,8 @@ namespace swmi
     // Tile geometry of a lattice (see swmi_internal.h for the meaning of the fields).
     struct Geometry
     {
-        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0, ecol_stride = 0;
+        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0, ecol_stride = 0,
+                 halo_stride = 0;
     };
     Geometry GeometryOf(uint32_t n0, uint32_t n1, uint32_t tile_rows);
     // Tile height for tile_rows == 0: the autotuned value for this extent if one was measured in this
@@ -138,6 +139,8 @@ namespace swmi
         DeviceBuffer<uint16_t> edge_list;
         DeviceBuffer<uint32_t> edge_count;
         DeviceBuffer<uint16_t> ecol;
+        DeviceBuffer<uint32_t> fbits;
+        DeviceBuffer<uint8_t> halo;
         DeviceBuffer<uint8_t> bond_dump;
         DeviceBuffer<uint8_t> bond_in;
 
@@ -204,6 +207,8 @@ struct swmi_engine
     void DropGraphs();
     void LaunchCheck(const char* what);
     void EnqueueSweeps(uint32_t n);
+    // applies the pending flip of every lattice (engine stream), so that their spin arrays hold the chains
+    void EnqueueMaterialize();
     void EnqueueSample(uint32_t mode);
     // side streams and events for every class beyond the first
     void EnsureSideStreams();
@@ -212,5 +217,7 @@ struct swmi_engine
     // captured graphs when enabled.
     void Run(uint32_t n, uint32_t sep, int32_t sample_mode);
     void OneLattice(const swmi::Lattice& l, const uint8_t* bond_in);
+    // OneLattice, then applies lattice l's pending flip
+    void MaterializeOne(const swmi::Lattice& l);
     uint64_t TotalSites() const;
 };

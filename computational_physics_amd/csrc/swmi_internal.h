@@ -14,6 +14,18 @@
 //   edge_count uint32[4*blocks]           entries of each tile's list
 //   ecol     uint16[2*ecol_stride*blocks] labels of each block's first (side 0) and last (side 1) column,
 //                            one entry per lattice row of the block, for the cross-block merge
+//   fbits    uint32[8*twr*blocks]   per block: flip bit of each edge-reaching root, by block-local index
+//                            (written by k_resolve, read by the next sweep's k_bond_label)
+//   halo     uint8[(ecol_stride+64)*blocks] per block: the flipped spins of its first column (one per
+//                            lattice row) and of its first wave row, which its left and upper neighbours
+//                            read as halos while the block's own spins still wait for their flip
+//
+// Deferred flip. A sweep is k_bond_label, k_merge, k_resolve. k_resolve decides every cluster's flip but
+// leaves the spins as they were; the next sweep's k_bond_label flips each site as it loads it (from its
+// label and the block's flip bits) and stores the flipped spin. LatState::applied is the iteration at
+// which the spin array holds the chain's state; applied != iteration means the last sweep's flip is still
+// pending, and every reader of spins (observables, downloads, checkpoints, the inspection hooks) first
+// runs k_flip, which applies it.
 //
 // Tiling. A "wave row" is 64 consecutive row-major sites, which is exactly one lcg32 group:
 //   n0 >= 64 (n0 % 64 == 0): a wave row is 64 columns of one lattice row;
@@ -68,12 +80,15 @@ namespace swmi
         uint32_t n_blocks;       // tiles_x * blocks_y
         uint32_t flip_rule;      // SWMI_FLIP_PARITY (0) or SWMI_FLIP_RANDOM (1)
         uint32_t seed;           // the lattice's lcg32 seed (keys the random flip rule)
+        uint32_t* fbits;         // 8 * twr words per block
+        uint8_t* halo;           // halo_stride bytes per block: first column, then first wave row
+        uint32_t halo_stride;    // ecol_stride + 64
     };
 
     // Per-lattice device counters and FP64 moments (uint32 words; moments start at word kStMoments).
     // Sweep schedule: k_bond_label advances a lattice only while iteration < it_end (two lcg32 updates per
-    // sweep) and records the decision in `active`, which k_merge and k_flip of the same sweep read (the
-    // iteration itself changes inside k_flip). Conditional samples (sample_mode kSampleDue) are taken
+    // sweep) and records the decision in `active`, which k_merge and k_resolve of the same sweep read (the
+    // iteration itself changes inside k_resolve). Conditional samples (sample_mode kSampleDue) are taken
     // when iteration > last_sample, iteration > it_meas and (iteration - it_meas) % sep2 == 0, i.e. after
     // every n_sep-th sweep past the warmup of the lattice's own schedule.
     struct LatState
@@ -88,13 +103,14 @@ namespace swmi
         uint32_t sep2;           // 2 * n_sep
         uint32_t last_sample;    // iteration of the last sample (0: none)
         uint32_t flip_key;       // random flip rule: key of the current sweep (written by k_bond_label)
-        uint32_t pad[2];
+        uint32_t applied;        // iteration at which the spin array holds the chain's state
+        uint32_t pad;
         double moments[6];       // samples, sum e, sum e^2, sum |m|, sum m^2, sum m^4
         // followed by int64 series[2 * series_cap]
     };
     constexpr uint32_t kLatStateWords = sizeof(LatState) / 4;
     constexpr uint32_t kStIteration = 0, kStSamples = 1, kStErr = 3, kStItEnd = 4, kStActive = 5, kStItMeas = 6,
-                       kStSep2 = 7, kStLastSample = 8, kStFlipKey = 9, kStMoments = 12;
+                       kStSep2 = 7, kStLastSample = 8, kStFlipKey = 9, kStApplied = 10, kStMoments = 12;
     static_assert(offsetof(LatState, moments) == 4 * kStMoments, "LatState layout");
 
     enum SampleMode : uint32_t
@@ -103,12 +119,6 @@ namespace swmi
         kSampleDue = 1u      // only lattices whose schedule calls for a sample at this iteration
     };
 
-    enum KernelFlags : uint32_t
-    {
-        kFlip = 1u,          // flip spins by root parity
-        kWriteLabels = 2u,   // write canonical labels of every site into parent[]
-        kAdvance = 4u        // advance the lcg32 iteration count by 2 (end of a sweep)
-    };
 
     constexpr uint32_t kErrLabelRange = 1u;  // a label chase left [0, n) or met an unflagged root
 
@@ -118,8 +128,15 @@ namespace swmi
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
                          bool all_tw64, bool debug, hipStream_t stream);
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
-    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
-                    uint32_t flags, hipStream_t stream);
+    // end of a sweep: flip bits and halos of every block, then iteration += 2
+    void LaunchResolve(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
+    // apply a pending flip to the spin array (no-op for lattices without one)
+    void LaunchMaterialize(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
+    // canonical label (minimum site index) of every site into parent[] (inspection)
+    void LaunchWriteLabels(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
+                           hipStream_t stream);
+    // a lattice's spin array was replaced: no flip is pending
+    void LaunchSetApplied(uint32_t* state, hipStream_t stream);
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t sample_mode,
                            hipStream_t stream);
     void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, uint32_t sample_mode, hipStream_t stream);

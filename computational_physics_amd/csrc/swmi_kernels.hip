@@ -321,6 +321,7 @@ namespace swmi
         // word of its own in 64-127 (the target of LDS operations a lane has no use for, so that they need
         // no exec mask); narrow tiles keep their union queue here
         __shared__ uint32_t Q[kWavesPerBlock * kQw];
+        __shared__ uint32_t FB[kSites / 32];   // flip bits of a pending flip, by block-local root index
         __shared__ uint64_t s_first_B[kWavesPerBlock];
         __shared__ uint64_t AC[8];              // Numerical Recipes pairs (c << 32 | a) by selector
 
@@ -344,7 +345,6 @@ namespace swmi
         const uint32_t lx = lane & (tw - 1u);
         const uint32_t lrow = TW64 ? 0u : lane / tw;
         const uint32_t x = bl.sx * tw + lx;
-        const uint32_t x_right = (x + 1u == n0) ? 0u : x + 1u;
         const bool rbound = lx == tw - 1u;
         const bool injected = DBG && d.bond_in != nullptr;
         const uint32_t thr = d.threshold;
@@ -355,11 +355,16 @@ namespace swmi
         if (bl.lb == 0 && threadIdx.x == 0)
         {
             stp[kStActive] = act ? 1u : 0u;
-            if (d.flip_rule != 0u)
-                stp[kStFlipKey] = FlipKey(d.seed, it0);
+            if (act && d.flip_rule != 0u)
+                stp[kStFlipKey] = FlipKey(d.seed, it0);   // an inactive lattice keeps the key of its pending flip
         }
         if (!act)
             return;
+        // The previous sweep's flip is still pending (applied != iteration): it is applied here, each site
+        // as it is loaded, from its label and the block's flip bits; the flipped spins are stored as the
+        // rows are processed. That flip belongs to the sweep that started at iteration it0 - 2.
+        const bool pending = !injected && stp[kStApplied] != it0;
+        const uint32_t pkey = (pending && d.flip_rule != 0u) ? FlipKey(d.seed, it0 - 2u) : 0u;
         const uint32_t it1 = it0 + 1u;
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
@@ -376,6 +381,9 @@ namespace swmi
         const uint32_t span = (bl.nwr - 1u) * bl.stride + 64u;   // sites from the block's first to its last
         const Rsrc rs_spins = MakeRsrc(d.spins + bl.base, span);
         const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
+        const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * twr;
+        const gptr<const uint8_t> halo_g = Glob(static_cast<const uint8_t*>(d.halo));
         const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
         const bool last_wave = w + 1u == nwaves;
         uint32_t* const Qw = Q + w * kQw;
@@ -402,16 +410,57 @@ namespace swmi
         const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
         const uint64_t lanes_below = (1ull << lane) - 1ull;
 
-        __syncthreads();   // AC
+        // pending flip: the block's flip bits
+        if (pending)
+            for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
+                FB[i] = fbits_g[i];
+        // flip bit of a site of this block with label lab (its edge root's bit, or the rule applied to its
+        // block-local root; block bases and row strides are even, so q's parity is the root's)
+        const auto flip_of = [&](const uint32_t lab, const uint32_t fword) {
+            const uint32_t q = lab & 0x7FFFu;
+            if (lab & kEdgeFlag)
+                return (fword >> (q & 31u)) & 1u;
+            return d.flip_rule != 0u ? (Fmix32(SiteOf(bl.base, bl.stride, q) ^ pkey) >> 31) : (q & 1u);
+        };
+
+        // spins below the tile (down halo), needed last but fetched first. Under a pending flip, below the
+        // block the neighbour's halo holds the flipped spin; inside the block the spin and label of the site
+        // below are carried across the barrier in the lane's spare LDS word (writing them there needs the
+        // loaded values, so these loads complete before the wave that owns the site may store its flipped
+        // spin) and flipped after it, with the block's flip bits.
+        uint32_t halo_down = 0;
+        const bool carry = nrow > 0 && pending && !last_wave;
+        if (nrow > 0)
+        {
+            const uint32_t y_last = (bl.wyb + r_begin + nrow - 1u) * rpw + lrow;
+            if (!pending)
+                halo_down = spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
+            else if (carry)
+            {
+                const uint32_t below = (r_begin + nrow) * bl.stride + lx;
+                const uint32_t lab = __builtin_amdgcn_raw_buffer_load_b16(rs_local, below * 2u, 0, 0);
+                const uint32_t sv = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, below, 0, 0);
+                __hip_atomic_store(dummy, (lab << 8) | (sv & 0xFFu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            else
+            {
+                const uint32_t sy_b = bl.sy + 1u == d.blocks_y ? 0u : bl.sy + 1u;
+                halo_down = halo_g[(sy_b * d.tiles_x + bl.sx) * d.halo_stride + d.ecol_stride + lx];
+            }
+        }
+        __syncthreads();   // AC, FB, the carried site below
+        if (carry)
+        {
+            const uint32_t v = LdsLoad(dummy);
+            const uint32_t lab = v >> 8;
+            halo_down = (v & 0xFFu) ^ flip_of(lab, LdsLoad(&FB[(lab & 0x7FFFu) >> 5]));
+        }
 
         uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
         uint32_t qlen = 0;
         uint32_t Rp = 0;   // 64-wide tiles: root of this lane's site in the previous row
         if (nrow > 0)
         {
-            // spins below the tile (down halo), needed last but fetched first
-            const uint32_t y_last = (bl.wyb + r_begin + nrow - 1u) * rpw + lrow;
-            const uint32_t halo_down = spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
 
             uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
             uint32_t prev_site = 0;
@@ -429,14 +478,17 @@ namespace swmi
                 uint32_t h = 0;
                 if (TW64)
                 {
-                    const uint32_t col = (bl.sx * 64u + 64u == n0) ? 0u : bl.sx * 64u + 64u;
-                    h = spins[(bl.wyb + lrow_ch) * n0 + col];
+                    const uint32_t sx_r = bl.sx + 1u == d.tiles_x ? 0u : bl.sx + 1u;
+                    if (pending)   // the right neighbour's flipped first column
+                        h = halo_g[(bl.sy * d.tiles_x + sx_r) * d.halo_stride + lrow_ch];
+                    else
+                        h = spins[(bl.wyb + lrow_ch) * n0 + sx_r * 64u];
                 }
                 // rotation offsets of the chunk's groups, needed on rotating sweeps only
                 uint32_t g_rot = 0;
                 if (kRot)
                     g_rot = rotp[(bl.base + lrow_ch * bl.stride) >> 6];
-                uint32_t sp[kCh], aux[kCh], rsh[kCh];
+                uint32_t sp[kCh], aux[kCh], lab[kCh];
 #pragma unroll
                 for (uint32_t i = 0; i < kCh; ++i)
                 {
@@ -445,7 +497,13 @@ namespace swmi
                     const uint32_t rowoff = Uniform(wr * bl.stride);
                     sp[i] = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, lane, rowoff, 0);
                     aux[i] = injected ? (bond_in + bl.base + rowoff)[lane] : __builtin_amdgcn_raw_buffer_load_b32(rs_rng, lane4, rowoff * 4u, 0);
-                    rsh[i] = TW64 ? 0u : spins[((bl.wyb + wr) * rpw + lrow) * n0 + x_right];
+                    lab[i] = pending ? __builtin_amdgcn_raw_buffer_load_b16(rs_local, lane * 2u, rowoff * 2u, 0) : 0u;
+                }
+                if (pending)
+                {
+#pragma unroll
+                    for (uint32_t i = 0; i < kCh; ++i)
+                        sp[i] ^= flip_of(lab[i], LdsLoad(&FB[(lab[i] & 0x7FFFu) >> 5]));
                 }
                 const uint64_t HR = TW64 ? __ballot((h & 1u) != 0u) : 0ull;
 
@@ -488,11 +546,14 @@ namespace swmi
                                 if (lane == 0)
                                     rotp[site >> 6] = (ReadLane(g_rot, i) + shift) & 63u;
                             }
+                            else if (pending)
+                                __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i]), rs_spins, lane, Uniform(wr * bl.stride), 0);
                             const uint64_t U1 = __ballot(u1 < thr);
                             U2 = __ballot(st < thr);
-                            // right neighbour's spin: next lane, or the halo for the last column
+                            // right neighbour's spin: next lane, or for the last column the halo (64-wide
+                            // tiles) or the first column of the same lattice row (narrow tiles: n0 == tw)
                             const uint64_t RS = TW64 ? ((S >> 1) | (((HR >> i) & 1ull) << 63))
-                                                     : (((S >> 1) & in_right) | __ballot(rbound && (rsh[i] & 1u) != 0u));
+                                                     : (((S >> 1) & in_right) | ((S << (tw - 1u)) & ~in_right));
                             Ball = ~(S ^ RS) & U1;
                         }
                         else
@@ -745,7 +806,6 @@ namespace swmi
         // hardware drops that store (no exec mask). A tile lists at most min(tile sites, block boundary
         // sites) roots, which is the list's capacity.
         const uint32_t cap = d.edge_cap;
-        const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
         const Rsrc rs_parent = MakeRsrc(d.parent + bl.base, span * 4u);
         const Rsrc rs_elist = MakeRsrc(d.edge_list + (kWavesPerBlock * bl.lb + w) * cap, cap * 2u);
         const Rsrc rs_ecol = MakeRsrc(d.ecol + 2u * bl.lb * d.ecol_stride, 4u * d.ecol_stride);
@@ -905,29 +965,81 @@ namespace swmi
     }
 
     // =============================================================================================
-    // k_flip: resolve + flip (reference ResolveLabels_Kernel / FlipClusters_Kernel,
-    // swendsen_wang_amd_gpu.cpp:403-467)
+    // k_resolve / k_flip: resolve + flip (reference ResolveLabels_Kernel / FlipClusters_Kernel,
+    // swendsen_wang_amd_gpu.cpp:403-467), split into the decision (k_resolve, end of every sweep) and its
+    // application to the spin array (the next sweep's k_bond_label, or k_flip when something reads spins)
     // =============================================================================================
-    // Sweep kernel. Two waves per block, so that every block of a 4096^2 lattice is resident at once and
-    // the latency of phase 1 is paid once per launch. (1) The roots of the block's edge-reaching clusters
-    // (the tile lists written by k_bond_label) are chased in the global parent array, one root per
-    // thread; the flip bit of the canonical label found goes into an LDS bitmap at the root's block-local
-    // index. (2) The block is streamed four sites per lane: a site flips by that bit for an edge cluster,
-    // or by the bit of its block-local root's site index otherwise. The first chunk of phase 2 is loaded
-    // before phase 1 so that its memory latency overlaps the chases.
     __device__ __forceinline__ uint32_t FlipBit(const uint32_t root, const bool random_rule, const uint32_t key)
     {
         return random_rule ? (Fmix32(root ^ key) >> 31) : (root & 1u);
     }
 
-    __global__ __launch_bounds__(128) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
-                                                  const uint32_t total_blocks)
+    // Flip bit of a site of block bl with label lab: its edge root's bit from fb, or the rule applied to
+    // its block-local root (block bases and row strides are even, so q's parity is its site's parity).
+    __device__ __forceinline__ uint32_t FlipOfLabel(const uint32_t lab, const uint32_t* fb, const Block& bl, const bool random_rule,
+                                                    const uint32_t key, const uint32_t qmax, bool& ok)
+    {
+        uint32_t q = lab & 0x7FFFu;
+        if (q >= qmax)
+        {
+            ok = false;
+            q = 0;
+        }
+        if (lab & kEdgeFlag)
+            return (fb[q >> 5] >> (q & 31u)) & 1u;
+        return random_rule ? FlipBit(SiteOf(bl.base, bl.stride, q), true, key) : (q & 1u);
+    }
+
+    // Phase 1 of the resolve: the roots of the block's edge-reaching clusters (the tile lists written by
+    // k_bond_label) are chased in the global parent array, one root per thread, and the flip bit of the
+    // canonical label found is set in fb at the root's block-local index (fb cleared by the caller).
+    template <uint32_t kThreads>
+    __device__ __forceinline__ void EdgeRootBits(const LatDesc& d, const Block& bl, uint32_t* fb, const bool random_rule,
+                                                 const uint32_t key, const uint32_t qmax, bool& ok)
+    {
+        const gptr<uint32_t> L = Glob(d.parent);
+        const uint32_t cap = d.edge_cap;
+        const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
+        const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
+        const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
+        const gptr<uint16_t> elist = Glob(d.edge_list) + kWavesPerBlock * bl.lb * cap;
+        for (uint32_t i = threadIdx.x; i < total; i += kThreads)
+        {
+            const uint32_t j = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
+            const uint32_t first = j == 0u ? 0u : (j == 1u ? e1 : (j == 2u ? e2 : e3));
+            const uint32_t q = elist[j * cap + (i - first)];
+            if (q >= qmax)
+            {
+                ok = false;
+                continue;
+            }
+            uint32_t x = SiteOf(bl.base, bl.stride, q);
+            uint32_t p = CachedLoad(&L[x]);
+            while (p != x)
+            {
+                if (p > x)
+                {
+                    ok = false;
+                    break;
+                }
+                x = p;
+                p = CachedLoad(&L[x]);
+            }
+            if (FlipBit(x, random_rule, key))
+                (void)__hip_atomic_fetch_or(&fb[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+
+    // End of a sweep, per block (two waves): phase 1 above, then the block's flip bits and the flipped
+    // spins of its first column and first wave row (the halos its left and upper neighbours read in the
+    // next sweep) are written out, and the lattice's counters move on: applied = the iteration whose
+    // spins the array holds, iteration += 2. The spin array itself is left for the next sweep to flip.
+    __global__ __launch_bounds__(128) void k_resolve(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                     const uint32_t total_blocks)
     {
         constexpr uint32_t kThreads = 128;
-        constexpr uint32_t kPass = kThreads / 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
-        constexpr uint32_t kCh = 8;                 // passes loaded together
         constexpr uint32_t kMaxSites = kWavesPerBlock * 64u * 64u;
-        __shared__ uint32_t fbits[kMaxSites / 32];
+        __shared__ uint32_t fb[kMaxSites / 32];
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
@@ -936,7 +1048,86 @@ namespace swmi
         if (stp[kStActive] == 0u)
             return;
         const Block bl = DecodeBlock(d, b);
-        const gptr<uint32_t> L = Glob(d.parent);
+        const uint32_t qmax = bl.nwr * 64u;
+        const bool random_rule = d.flip_rule != 0u;
+        const uint32_t key = random_rule ? stp[kStFlipKey] : 0u;
+        const gptr<const uint8_t> spins = Glob(static_cast<const uint8_t*>(d.spins));
+        const gptr<const uint16_t> local = Glob(static_cast<const uint16_t*>(d.local));
+        const gptr<const uint16_t> ecol0 = Glob(static_cast<const uint16_t*>(d.ecol)) + 2u * bl.lb * d.ecol_stride;
+        bool ok = true;
+
+        // halo sites: entry t < rows is the first column at lattice row t of the block, entry rows + j the
+        // first wave row at lane j. Their spins and labels are fetched before the chases.
+        const uint32_t rows = bl.nwr * d.rpw;
+        // rows <= kWavesPerBlock * 64 (a tile has twr <= tw wave rows of rpw = 64 / tw lattice rows)
+        constexpr uint32_t kHaloPer = (kWavesPerBlock * 64u + 64u + kThreads - 1u) / kThreads;
+        uint32_t hs[kHaloPer], hl[kHaloPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k)
+        {
+            const uint32_t t = threadIdx.x + k * kThreads;
+            hs[k] = 0;
+            hl[k] = 0;
+            if (t < rows)
+            {
+                hs[k] = spins[(bl.wyb * d.rpw + t) * d.n0 + bl.sx * d.tw];
+                hl[k] = ecol0[t];
+            }
+            else if (t < rows + 64u)
+            {
+                hs[k] = spins[bl.base + (t - rows)];
+                hl[k] = local[bl.base + (t - rows)];
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
+            fb[i] = 0u;
+        __syncthreads();
+        EdgeRootBits<kThreads>(d, bl, fb, random_rule, key, qmax, ok);
+        __syncthreads();
+
+        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * 8u * d.twr;
+        for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
+            fbits_g[i] = fb[i];
+        const gptr<uint8_t> halo = Glob(d.halo) + bl.lb * d.halo_stride;
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k)
+        {
+            const uint32_t t = threadIdx.x + k * kThreads;
+            if (t < rows + 64u)
+            {
+                const uint32_t v = (hs[k] ^ FlipOfLabel(hl[k], fb, bl, random_rule, key, qmax, ok)) & 1u;
+                halo[t < rows ? t : d.ecol_stride + (t - rows)] = static_cast<uint8_t>(v);
+            }
+        }
+        if (!ok)
+            (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bl.lb == 0 && threadIdx.x == 0)
+        {
+            const uint32_t it = stp[kStIteration];
+            stp[kStApplied] = it;
+            stp[kStIteration] = it + 2u;
+        }
+    }
+
+    // Applies a pending flip to the spin array (lattices without one return at once): per block (two
+    // waves, every block of a 4096^2 lattice resident at once), the flip bits written by k_resolve and the
+    // labels of the last sweep, four sites per lane. k_set_applied follows it.
+    __global__ __launch_bounds__(128) void k_flip(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                  const uint32_t total_blocks)
+    {
+        constexpr uint32_t kThreads = 128;
+        constexpr uint32_t kPass = kThreads / 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
+        constexpr uint32_t kCh = 8;                 // passes loaded together
+        constexpr uint32_t kMaxSites = kWavesPerBlock * 64u * 64u;
+        __shared__ uint32_t fb[kMaxSites / 32];
+        const uint32_t b = RemapBlock(total_blocks);
+        if (b >= total_blocks)
+            return;
+        const LatDesc& d = descs[block_lat[b]];
+        const gptr<uint32_t> stp = Glob(d.state);
+        if (stp[kStApplied] == stp[kStIteration])
+            return;
+        const Block bl = DecodeBlock(d, b);
         const gptr<uint16_t> local = Glob(d.local);
         const gptr<uint8_t> spins = Glob(d.spins);
         const uint32_t qmax = bl.nwr * 64u;
@@ -964,47 +1155,11 @@ namespace swmi
             }
         };
         load(0u);
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * d.twr;
         for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
-            fbits[i] = 0u;
+            fb[i] = fbits_g[i];
         __syncthreads();
 
-        // phase 1: flip bits of the canonical labels of the edge-reaching roots
-        {
-            const uint32_t cap = d.edge_cap;
-            const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
-            const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
-            const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
-            const gptr<uint16_t> elist = Glob(d.edge_list) + kWavesPerBlock * bl.lb * cap;
-            for (uint32_t i = threadIdx.x; i < total; i += kThreads)
-            {
-                const uint32_t j = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
-                const uint32_t first = j == 0u ? 0u : (j == 1u ? e1 : (j == 2u ? e2 : e3));
-                const uint32_t q = elist[j * cap + (i - first)];
-                if (q >= qmax)
-                {
-                    ok = false;
-                    continue;
-                }
-                uint32_t x = SiteOf(bl.base, bl.stride, q);
-                uint32_t p = CachedLoad(&L[x]);
-                while (p != x)
-                {
-                    if (p > x)
-                    {
-                        ok = false;
-                        break;
-                    }
-                    x = p;
-                    p = CachedLoad(&L[x]);
-                }
-                if (FlipBit(x, random_rule, key))
-                    (void)__hip_atomic_fetch_or(&fbits[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-        __syncthreads();
-
-        // phase 2: flips, four sites per lane. Block bases and row strides are even, so the parity of a
-        // block-local root index is the parity of its site index.
         for (uint32_t r0 = 0;;)
         {
 #pragma unroll
@@ -1018,18 +1173,7 @@ namespace swmi
                     uint32_t flip = 0;
 #pragma unroll
                     for (uint32_t e = 0; e < 4; ++e)
-                    {
-                        uint32_t q = lab[e] & 0x7FFFu;
-                        const bool edge = (lab[e] & kEdgeFlag) != 0u;
-                        if (q >= qmax)
-                        {
-                            ok = false;
-                            q = 0;
-                        }
-                        const uint32_t own = random_rule ? FlipBit(SiteOf(bl.base, bl.stride, q), true, key) : (q & 1u);
-                        const uint32_t bit = edge ? ((LdsLoad(&fbits[q >> 5]) >> (q & 31u)) & 1u) : own;
-                        flip |= bit << (8u * e);
-                    }
+                        flip |= FlipOfLabel(lab[e], fb, bl, random_rule, key, qmax, ok) << (8u * e);
                     *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;
                 }
             }
@@ -1040,14 +1184,29 @@ namespace swmi
         }
         if (!ok)
             (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (bl.lb == 0 && threadIdx.x == 0)
-            stp[kStIteration] += 2u;
     }
 
-    // Inspection kernel (labels of every site, no flips): the same two phases with the canonical label
-    // itself kept in LDS (one word per block site) and written to parent[] for every site.
+    // After k_flip over a work list: every lattice's spin array holds its chain's state.
+    __global__ __launch_bounds__(256) void k_set_applied(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
+                                                         const uint32_t total_blocks)
+    {
+        const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+        if (b >= total_blocks)
+            return;
+        const LatDesc& d = descs[block_lat[b]];
+        if (b == d.block_base)
+            d.state[kStApplied] = d.state[kStIteration];
+    }
+
+    __global__ void k_set_applied_one(uint32_t* __restrict__ state)
+    {
+        state[kStApplied] = state[kStIteration];
+    }
+
+    // Inspection kernel: the canonical label (minimum site index) of every site, written to parent[]. The
+    // same phase 1 with the canonical label itself kept in LDS (one word per block site).
     __global__ __launch_bounds__(256) void k_write_labels(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
-                                                          const uint32_t total_blocks, const uint32_t flags)
+                                                          const uint32_t total_blocks)
     {
         constexpr uint32_t kPass = 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
         constexpr uint32_t kCh = 4;      // passes loaded together
@@ -1056,37 +1215,23 @@ namespace swmi
         if (b >= total_blocks)
             return;
         const LatDesc& d = descs[block_lat[b]];
-        const bool do_flip = (flags & kFlip) != 0u;
-        if (do_flip && Glob(d.state)[kStActive] == 0u)
-            return;
         const Block bl = DecodeBlock(d, b);
         const gptr<uint32_t> L = Glob(d.parent);
         const gptr<uint16_t> local = Glob(d.local);
-        const gptr<uint8_t> spins = Glob(d.spins);
         const uint32_t qmax = bl.nwr * 64u;
-        const bool write_labels = (flags & kWriteLabels) != 0u;
-        const bool random_rule = d.flip_rule != 0u;
-        const uint32_t key = random_rule ? Glob(d.state)[kStFlipKey] : 0u;
         const uint32_t sub = threadIdx.x >> 4;
         const uint32_t col = (threadIdx.x & 15u) * 4u;
         bool ok = true;
 
         u32x2 lv[kCh];
-        uint32_t sv[kCh];
         const auto load = [&](const uint32_t r0) {
 #pragma unroll
             for (uint32_t k = 0; k < kCh; ++k)
             {
                 const uint32_t r = r0 + sub + k * kPass;
                 lv[k] = u32x2{0u, 0u};
-                sv[k] = 0u;
                 if (r < bl.nwr)
-                {
-                    const uint32_t site = bl.base + r * bl.stride + col;
-                    lv[k] = *(gptr<const u32x2>)(local + site);
-                    if (do_flip)
-                        sv[k] = *(gptr<const uint32_t>)(spins + site);
-                }
+                    lv[k] = *(gptr<const u32x2>)(local + bl.base + r * bl.stride + col);
             }
         };
         load(0u);
@@ -1125,7 +1270,7 @@ namespace swmi
         }
         __syncthreads();
 
-        // phase 2: labels and flips, four sites per lane
+        // phase 2: every site's label, four sites per lane
         for (uint32_t r0 = 0;;)
         {
 #pragma unroll
@@ -1137,7 +1282,6 @@ namespace swmi
                     const uint32_t site = bl.base + r * bl.stride + col;
                     const uint32_t lab[4] = {lv[k].x & 0xFFFFu, lv[k].x >> 16, lv[k].y & 0xFFFFu, lv[k].y >> 16};
                     uint32_t root[4];
-                    uint32_t flip = 0;
 #pragma unroll
                     for (uint32_t e = 0; e < 4; ++e)
                     {
@@ -1150,13 +1294,8 @@ namespace swmi
                         }
                         const uint32_t g = R[edge ? q : 0u];
                         root[e] = edge ? g : SiteOf(bl.base, bl.stride, q);
-                        const uint32_t bit = random_rule ? (Fmix32(root[e] ^ key) >> 31) : (root[e] & 1u);
-                        flip |= bit << (8u * e);
                     }
-                    if (do_flip)
-                        *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;
-                    if (write_labels)
-                        *(gptr<u32x4>)(L + site) = u32x4{root[0], root[1], root[2], root[3]};
+                    *(gptr<u32x4>)(L + site) = u32x4{root[0], root[1], root[2], root[3]};
                 }
             }
             r0 += kCh * kPass;
@@ -1166,8 +1305,6 @@ namespace swmi
         }
         if (!ok)
             (void)__hip_atomic_fetch_or(Glob(d.state) + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((flags & kAdvance) && bl.lb == 0 && threadIdx.x == 0)
-            Glob(d.state)[kStIteration] += 2u;
     }
 
     // =============================================================================================
@@ -1443,17 +1580,25 @@ namespace swmi
         k_merge<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
-    void LaunchFlip(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
-                    const uint32_t flags, hipStream_t stream)
+    void LaunchResolve(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
     {
-        if (flags == (kFlip | kAdvance))
-        {
-            k_flip<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
-            return;
-        }
-        const size_t lds = static_cast<size_t>(kWavesPerBlock) * max_twr * 64u * sizeof(uint32_t);
-        k_write_labels<<<XcdGrid(total_blocks), 256, lds, stream>>>(d_desc, d_block_lat, total_blocks, flags);
+        k_resolve<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
+
+    void LaunchMaterialize(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
+    {
+        k_flip<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+        k_set_applied<<<(total_blocks + 255u) / 256u, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+    }
+
+    void LaunchWriteLabels(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
+                           hipStream_t stream)
+    {
+        const size_t lds = static_cast<size_t>(kWavesPerBlock) * max_twr * 64u * sizeof(uint32_t);
+        k_write_labels<<<XcdGrid(total_blocks), 256, lds, stream>>>(d_desc, d_block_lat, total_blocks);
+    }
+
+    void LaunchSetApplied(uint32_t* state, hipStream_t stream) { k_set_applied_one<<<1, 1, 0, stream>>>(state); }
 
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t sample_mode,
                            hipStream_t stream)

@@ -182,7 +182,8 @@ int swmi_get_error_flags(swmi_engine* engine, uint32_t lattice, uint32_t* flags)
 
 /* ---- measurement support ---- */
 /* Average device time per kernel over n_sweeps sweeps, measured with HIP events on the engine stream:
- * ms[0] bond+label, ms[1] merge, ms[2] flip. */
+ * ms[0] bond+label (which also applies the previous sweep's flip), ms[1] merge, ms[2] resolve (the flip
+ * decision of every cluster). */
 int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms);
 /* Device-to-device copy of `bytes` bytes (calibration kernel for HBM byte counters). */
 int swmi_copy_calibration(size_t bytes, uint32_t repeats, float* ms);
