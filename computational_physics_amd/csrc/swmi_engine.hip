@@ -140,7 +140,7 @@ namespace swmi
         return n * (1 + 4 + 2 + 4) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
                4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
                static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4) + 4ull * g.ecol_stride * g.n_blocks +
-               32ull * g.twr * g.n_blocks + static_cast<uint64_t>(g.halo_stride) * g.n_blocks;
+               32ull * g.twr * g.n_blocks + 2ull * g.halo_stride * g.n_blocks;
     }
 
     bool Lattice::SameShape(const swmi_lattice_config& c) const
@@ -176,7 +176,7 @@ namespace swmi
         edge_count.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
         ecol.allocate(2ull * g.ecol_stride * g.n_blocks);
         fbits.allocate(8ull * g.twr * g.n_blocks);
-        halo.allocate(static_cast<size_t>(g.halo_stride) * g.n_blocks);
+        halo.allocate(2ull * g.halo_stride * g.n_blocks);
     }
 
     void Lattice::Initialize(const swmi_lattice_config& c, hipStream_t stream)

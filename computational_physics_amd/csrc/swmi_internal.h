@@ -16,9 +16,11 @@
 //                            one entry per lattice row of the block, for the cross-block merge
 //   fbits    uint32[8*twr*blocks]   per block: flip bit of each edge-reaching root, by block-local index
 //                            (written by k_resolve, read by the next sweep's k_bond_label)
-//   halo     uint8[(ecol_stride+64)*blocks] per block: the flipped spins of its first column (one per
-//                            lattice row) and of its first wave row, which its left and upper neighbours
-//                            read as halos while the block's own spins still wait for their flip
+//   halo     uint8[2*(ecol_stride+64)*blocks] two sides (by sweep parity) of, per block, the spins of its
+//                            first column (one per lattice row) and of its first wave row: k_bond_label
+//                            writes them into the sweep's side, k_resolve flips them in place, and in the
+//                            next sweep the block's left and upper neighbours read them as halos while the
+//                            block's own spins still wait for their flip
 //
 // Deferred flip. A sweep is k_bond_label, k_merge, k_resolve. k_resolve decides every cluster's flip but
 // leaves the spins as they were; the next sweep's k_bond_label flips each site as it loads it (from its
@@ -81,7 +83,7 @@ namespace swmi
         uint32_t flip_rule;      // SWMI_FLIP_PARITY (0) or SWMI_FLIP_RANDOM (1)
         uint32_t seed;           // the lattice's lcg32 seed (keys the random flip rule)
         uint32_t* fbits;         // 8 * twr words per block
-        uint8_t* halo;           // halo_stride bytes per block: first column, then first wave row
+        uint8_t* halo;           // two sides of n_blocks * halo_stride bytes: first column, then first wave row
         uint32_t halo_stride;    // ecol_stride + 64
     };
 
@@ -104,13 +106,14 @@ namespace swmi
         uint32_t last_sample;    // iteration of the last sample (0: none)
         uint32_t flip_key;       // random flip rule: key of the current sweep (written by k_bond_label)
         uint32_t applied;        // iteration at which the spin array holds the chain's state
-        uint32_t pad;
+        uint32_t halo_side;      // side of the halo array the current sweep writes ((iteration / 2) & 1)
         double moments[6];       // samples, sum e, sum e^2, sum |m|, sum m^2, sum m^4
         // followed by int64 series[2 * series_cap]
     };
     constexpr uint32_t kLatStateWords = sizeof(LatState) / 4;
     constexpr uint32_t kStIteration = 0, kStSamples = 1, kStErr = 3, kStItEnd = 4, kStActive = 5, kStItMeas = 6,
-                       kStSep2 = 7, kStLastSample = 8, kStFlipKey = 9, kStApplied = 10, kStMoments = 12;
+                       kStSep2 = 7, kStLastSample = 8, kStFlipKey = 9, kStApplied = 10, kStHaloSide = 11,
+                       kStMoments = 12;
     static_assert(offsetof(LatState, moments) == 4 * kStMoments, "LatState layout");
 
     enum SampleMode : uint32_t

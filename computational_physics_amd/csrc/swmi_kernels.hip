@@ -357,6 +357,8 @@ namespace swmi
             stp[kStActive] = act ? 1u : 0u;
             if (act && d.flip_rule != 0u)
                 stp[kStFlipKey] = FlipKey(d.seed, it0);   // an inactive lattice keeps the key of its pending flip
+            if (act && !injected)
+                stp[kStHaloSide] = (it0 >> 1) & 1u;
         }
         if (!act)
             return;
@@ -383,7 +385,12 @@ namespace swmi
         const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
         const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
         const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * twr;
-        const gptr<const uint8_t> halo_g = Glob(static_cast<const uint8_t*>(d.halo));
+        // halos: this sweep writes the block's first column and first wave row into its side of the halo
+        // array; the other side holds the neighbours' flipped halos of the pending flip
+        const uint32_t hside = (it0 >> 1) & 1u;
+        const gptr<const uint8_t> halo_g = Glob(static_cast<const uint8_t*>(d.halo)) + (hside ^ 1u) * d.n_blocks * d.halo_stride;
+        const Rsrc rs_halo = MakeRsrc(d.halo + (hside * d.n_blocks + bl.lb) * d.halo_stride, d.halo_stride);
+        const uint32_t hcol_off = (lx == 0u && !injected) ? lrow : kNoStore;   // first column: one lane per lattice row
         const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
         const bool last_wave = w + 1u == nwaves;
         uint32_t* const Qw = Q + w * kQw;
@@ -516,6 +523,9 @@ namespace swmi
                         const uint32_t wr = r_begin + r;
                         const uint32_t site = bl.base + wr * bl.stride + lane;
                         const uint64_t S = __ballot((sp[i] & 1u) != 0u);
+                        __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i] & 1u), rs_halo, hcol_off, wr * rpw, 0);
+                        if (wr == 0 && !injected)
+                            __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i] & 1u), rs_halo, d.ecol_stride + lane, 0, 0);
                         uint64_t Ball, U2;
                         if (!injected)
                         {
@@ -1051,13 +1061,14 @@ namespace swmi
         const uint32_t qmax = bl.nwr * 64u;
         const bool random_rule = d.flip_rule != 0u;
         const uint32_t key = random_rule ? stp[kStFlipKey] : 0u;
-        const gptr<const uint8_t> spins = Glob(static_cast<const uint8_t*>(d.spins));
         const gptr<const uint16_t> local = Glob(static_cast<const uint16_t*>(d.local));
         const gptr<const uint16_t> ecol0 = Glob(static_cast<const uint16_t*>(d.ecol)) + 2u * bl.lb * d.ecol_stride;
+        const gptr<uint8_t> halo = Glob(d.halo) + (stp[kStHaloSide] * d.n_blocks + bl.lb) * d.halo_stride;
         bool ok = true;
 
         // halo sites: entry t < rows is the first column at lattice row t of the block, entry rows + j the
-        // first wave row at lane j. Their spins and labels are fetched before the chases.
+        // first wave row at lane j. Their spins (written into the halo by this sweep's k_bond_label) and
+        // labels are fetched before the chases; the halo is flipped in place.
         const uint32_t rows = bl.nwr * d.rpw;
         // rows <= kWavesPerBlock * 64 (a tile has twr <= tw wave rows of rpw = 64 / tw lattice rows)
         constexpr uint32_t kHaloPer = (kWavesPerBlock * 64u + 64u + kThreads - 1u) / kThreads;
@@ -1070,12 +1081,12 @@ namespace swmi
             hl[k] = 0;
             if (t < rows)
             {
-                hs[k] = spins[(bl.wyb * d.rpw + t) * d.n0 + bl.sx * d.tw];
+                hs[k] = halo[t];
                 hl[k] = ecol0[t];
             }
             else if (t < rows + 64u)
             {
-                hs[k] = spins[bl.base + (t - rows)];
+                hs[k] = halo[d.ecol_stride + (t - rows)];
                 hl[k] = local[bl.base + (t - rows)];
             }
         }
@@ -1088,7 +1099,6 @@ namespace swmi
         const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * 8u * d.twr;
         for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
             fbits_g[i] = fb[i];
-        const gptr<uint8_t> halo = Glob(d.halo) + bl.lb * d.halo_stride;
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k)
         {
