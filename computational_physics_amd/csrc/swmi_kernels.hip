@@ -366,7 +366,6 @@ namespace swmi
         // as it is loaded, from its label and the block's flip bits; the flipped spins are stored as the
         // rows are processed. That flip belongs to the sweep that started at iteration it0 - 2.
         const bool pending = !injected && stp[kStApplied] != it0;
-        const uint32_t pkey = (pending && d.flip_rule != 0u) ? FlipKey(d.seed, it0 - 2u) : 0u;
         const uint32_t it1 = it0 + 1u;
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
@@ -417,18 +416,12 @@ namespace swmi
         const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
         const uint64_t lanes_below = (1ull << lane) - 1ull;
 
-        // pending flip: the block's flip bits
+        // pending flip: the block's flip bits, one per block-local root index (k_resolve wrote the bit of
+        // every root, edge-reaching or not), so a site's flip is the bit at its label's root index
         if (pending)
             for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
                 FB[i] = fbits_g[i];
-        // flip bit of a site of this block with label lab (its edge root's bit, or the rule applied to its
-        // block-local root; block bases and row strides are even, so q's parity is the root's)
-        const auto flip_of = [&](const uint32_t lab, const uint32_t fword) {
-            const uint32_t q = lab & 0x7FFFu;
-            if (lab & kEdgeFlag)
-                return (fword >> (q & 31u)) & 1u;
-            return d.flip_rule != 0u ? (Fmix32(SiteOf(bl.base, bl.stride, q) ^ pkey) >> 31) : (q & 1u);
-        };
+        const auto flip_of = [&](const uint32_t lab) { return (LdsLoad(&FB[(lab & 0x7FFFu) >> 5]) >> (lab & 31u)) & 1u; };
 
         // spins below the tile (down halo), needed last but fetched first. Under a pending flip, below the
         // block the neighbour's halo holds the flipped spin; inside the block the spin and label of the site
@@ -460,7 +453,7 @@ namespace swmi
         {
             const uint32_t v = LdsLoad(dummy);
             const uint32_t lab = v >> 8;
-            halo_down = (v & 0xFFu) ^ flip_of(lab, LdsLoad(&FB[(lab & 0x7FFFu) >> 5]));
+            halo_down = (v & 0xFFu) ^ flip_of(lab);
         }
 
         uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
@@ -472,15 +465,20 @@ namespace swmi
             uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
             uint32_t prev_site = 0;
             // the chunk loop exists twice: sweeps without a group rotation (13 in 15) carry no rotation code
-            const auto chunks = [&](auto rotating) {
+            // (and twice more for the sweeps that apply a pending flip on load, the rule after any sweep)
+            uint32_t colv = 0;   // 64-wide tiles: lane r collects the first-column spin of row r
+            const auto chunks = [&](auto rotating, auto flipping) {
             constexpr bool kRot = decltype(rotating)::value;
-            // a tile of at most kCh rows is one chunk, starting at row 0
-            for (uint32_t r0 = 0; r0 < (TWR_MAX <= kCh ? 1u : nrow); r0 += kCh)
+            constexpr bool kPend = decltype(flipping)::value;
+            // rows loaded together: a flip on load also keeps the labels in flight, so half as many
+            constexpr uint32_t kC = kPend && kCh > 1u ? kCh / 2u : kCh;
+            // a tile of at most kC rows is one chunk, starting at row 0
+            for (uint32_t r0 = 0; r0 < (TWR_MAX <= kC ? 1u : nrow); r0 += kC)
             {
                 // The loads carry no condition (rows past the tile re-read its last row, values unused), so
                 // that they issue back to back and their latencies overlap.
                 // lane i: row r0 + i of the chunk (for the per-row values fetched one per lane)
-                const uint32_t lrow_ch = r_begin + min(r0 + (lane % kCh), nrow - 1u);
+                const uint32_t lrow_ch = r_begin + min(r0 + (lane % kC), nrow - 1u);
                 // right halo of a 64-wide tile: lane i fetches the column-64 spin of row r0 + i
                 uint32_t h = 0;
                 if (TW64)
@@ -495,27 +493,27 @@ namespace swmi
                 uint32_t g_rot = 0;
                 if (kRot)
                     g_rot = rotp[(bl.base + lrow_ch * bl.stride) >> 6];
-                uint32_t sp[kCh], aux[kCh], lab[kCh];
+                uint32_t sp[kC], aux[kC], lab[kC];
 #pragma unroll
-                for (uint32_t i = 0; i < kCh; ++i)
+                for (uint32_t i = 0; i < kC; ++i)
                 {
                     // row base in scalar registers, the lane offset in a vector register
                     const uint32_t wr = r_begin + min(r0 + i, nrow - 1u);
                     const uint32_t rowoff = Uniform(wr * bl.stride);
                     sp[i] = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, lane, rowoff, 0);
                     aux[i] = injected ? (bond_in + bl.base + rowoff)[lane] : __builtin_amdgcn_raw_buffer_load_b32(rs_rng, lane4, rowoff * 4u, 0);
-                    lab[i] = pending ? __builtin_amdgcn_raw_buffer_load_b16(rs_local, lane * 2u, rowoff * 2u, 0) : 0u;
+                    lab[i] = kPend ? __builtin_amdgcn_raw_buffer_load_b16(rs_local, lane * 2u, rowoff * 2u, 0) : 0u;
                 }
-                if (pending)
+                if (kPend)
                 {
 #pragma unroll
-                    for (uint32_t i = 0; i < kCh; ++i)
-                        sp[i] ^= flip_of(lab[i], LdsLoad(&FB[(lab[i] & 0x7FFFu) >> 5]));
+                    for (uint32_t i = 0; i < kC; ++i)
+                        sp[i] ^= flip_of(lab[i]);
                 }
                 const uint64_t HR = TW64 ? __ballot((h & 1u) != 0u) : 0ull;
 
 #pragma unroll
-                for (uint32_t i = 0; i < kCh; ++i)
+                for (uint32_t i = 0; i < kC; ++i)
                 {
                     const uint32_t r = r0 + i;
                     if (r < nrow)
@@ -523,7 +521,10 @@ namespace swmi
                         const uint32_t wr = r_begin + r;
                         const uint32_t site = bl.base + wr * bl.stride + lane;
                         const uint64_t S = __ballot((sp[i] & 1u) != 0u);
-                        __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i] & 1u), rs_halo, hcol_off, wr * rpw, 0);
+                        if (TW64)
+                            colv = LaneBit(1ull << r) ? static_cast<uint32_t>(S) : colv;   // bit 0 of S: lane 0's spin
+                        else
+                            __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i] & 1u), rs_halo, hcol_off, wr * rpw, 0);
                         if (wr == 0 && !injected)
                             __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i] & 1u), rs_halo, d.ecol_stride + lane, 0, 0);
                         uint64_t Ball, U2;
@@ -556,7 +557,7 @@ namespace swmi
                                 if (lane == 0)
                                     rotp[site >> 6] = (ReadLane(g_rot, i) + shift) & 63u;
                             }
-                            else if (pending)
+                            else if (kPend)
                                 __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i]), rs_spins, lane, Uniform(wr * bl.stride), 0);
                             const uint64_t U1 = __ballot(u1 < thr);
                             U2 = __ballot(st < thr);
@@ -662,9 +663,18 @@ namespace swmi
             }
             };
             if (rot1 || rot2)
-                chunks(std::true_type{});
+            {
+                if (pending)
+                    chunks(std::true_type{}, std::true_type{});
+                else
+                    chunks(std::true_type{}, std::false_type{});
+            }
+            else if (pending)
+                chunks(std::false_type{}, std::true_type{});
             else
-                chunks(std::false_type{});
+                chunks(std::false_type{}, std::false_type{});
+            if (TW64 && !injected)   // first column of the tile's rows (rpw == 1): one store per wave
+                __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(colv & 1u), rs_halo, lane < nrow ? r_begin + lane : kNoStore, 0, 0);
 
             // bonds out of the last wave row: into the same wave row (tw < 64), into the next tile of the
             // block, or across the block edge
@@ -984,10 +994,22 @@ namespace swmi
         return random_rule ? (Fmix32(root ^ key) >> 31) : (root & 1u);
     }
 
-    // Flip bit of a site of block bl with label lab: its edge root's bit from fb, or the rule applied to
-    // its block-local root (block bases and row strides are even, so q's parity is its site's parity).
-    __device__ __forceinline__ uint32_t FlipOfLabel(const uint32_t lab, const uint32_t* fb, const Block& bl, const bool random_rule,
-                                                    const uint32_t key, const uint32_t qmax, bool& ok)
+    // A block's flip bitmap has one bit per block-local index q: for the root of a cluster that stays
+    // inside the block, the rule applied to the root's own site; for an edge-reaching root, the rule
+    // applied to its canonical label (set by EdgeRootBits). Word w of the rule part: bit j for q = 32w + j
+    // (block bases and row strides are even, so under the parity rule q's parity is its site's parity).
+    __device__ __forceinline__ uint32_t RuleWord(const Block& bl, const uint32_t w, const bool random_rule, const uint32_t key)
+    {
+        if (!random_rule)
+            return 0xAAAAAAAAu;
+        uint32_t v = 0;
+        for (uint32_t j = 0; j < 32u; ++j)
+            v |= FlipBit(SiteOf(bl.base, bl.stride, 32u * w + j), true, key) << j;
+        return v;
+    }
+
+    // flip bit of a site with label lab (root index, with the edge flag) from the block's bitmap
+    __device__ __forceinline__ uint32_t FlipOfLabel(const uint32_t lab, const uint32_t* fb, const uint32_t qmax, bool& ok)
     {
         uint32_t q = lab & 0x7FFFu;
         if (q >= qmax)
@@ -995,14 +1017,13 @@ namespace swmi
             ok = false;
             q = 0;
         }
-        if (lab & kEdgeFlag)
-            return (fb[q >> 5] >> (q & 31u)) & 1u;
-        return random_rule ? FlipBit(SiteOf(bl.base, bl.stride, q), true, key) : (q & 1u);
+        return (fb[q >> 5] >> (q & 31u)) & 1u;
     }
 
     // Phase 1 of the resolve: the roots of the block's edge-reaching clusters (the tile lists written by
-    // k_bond_label) are chased in the global parent array, one root per thread, and the flip bit of the
-    // canonical label found is set in fb at the root's block-local index (fb cleared by the caller).
+    // k_bond_label) are chased in the global parent array, one root per thread, and fb's bit at the root's
+    // block-local index (the rule applied to the root's own site, set by the caller) is turned into the rule
+    // applied to the canonical label found. Each root is listed once.
     template <uint32_t kThreads>
     __device__ __forceinline__ void EdgeRootBits(const LatDesc& d, const Block& bl, uint32_t* fb, const bool random_rule,
                                                  const uint32_t key, const uint32_t qmax, bool& ok)
@@ -1035,8 +1056,9 @@ namespace swmi
                 x = p;
                 p = CachedLoad(&L[x]);
             }
-            if (FlipBit(x, random_rule, key))
-                (void)__hip_atomic_fetch_or(&fb[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t own = FlipBit(SiteOf(bl.base, bl.stride, q), random_rule, key);
+            if (FlipBit(x, random_rule, key) != own)
+                (void)__hip_atomic_fetch_xor(&fb[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
 
@@ -1091,7 +1113,7 @@ namespace swmi
             }
         }
         for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
-            fb[i] = 0u;
+            fb[i] = RuleWord(bl, i, random_rule, key);
         __syncthreads();
         EdgeRootBits<kThreads>(d, bl, fb, random_rule, key, qmax, ok);
         __syncthreads();
@@ -1105,7 +1127,7 @@ namespace swmi
             const uint32_t t = threadIdx.x + k * kThreads;
             if (t < rows + 64u)
             {
-                const uint32_t v = (hs[k] ^ FlipOfLabel(hl[k], fb, bl, random_rule, key, qmax, ok)) & 1u;
+                const uint32_t v = (hs[k] ^ FlipOfLabel(hl[k], fb, qmax, ok)) & 1u;
                 halo[t < rows ? t : d.ecol_stride + (t - rows)] = static_cast<uint8_t>(v);
             }
         }
@@ -1141,8 +1163,6 @@ namespace swmi
         const gptr<uint16_t> local = Glob(d.local);
         const gptr<uint8_t> spins = Glob(d.spins);
         const uint32_t qmax = bl.nwr * 64u;
-        const bool random_rule = d.flip_rule != 0u;
-        const uint32_t key = random_rule ? stp[kStFlipKey] : 0u;
         const uint32_t sub = threadIdx.x >> 4;
         const uint32_t col = (threadIdx.x & 15u) * 4u;
         bool ok = true;
@@ -1183,7 +1203,7 @@ namespace swmi
                     uint32_t flip = 0;
 #pragma unroll
                     for (uint32_t e = 0; e < 4; ++e)
-                        flip |= FlipOfLabel(lab[e], fb, bl, random_rule, key, qmax, ok) << (8u * e);
+                        flip |= FlipOfLabel(lab[e], fb, qmax, ok) << (8u * e);
                     *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;
                 }
             }
