@@ -142,8 +142,20 @@ namespace swmi
     // than one hop from its root is pointed at the root (the atomic returns nothing, so the wave does not
     // wait for it). Then the larger root is linked under the smaller, and a lost race continues with the
     // value found (the loop of the reference Merge, swendsen_wang_amd_gpu.cpp:368-401).
+    // The ends start as the edge roots k_bond_label listed (parent = self), so the first link is tried at
+    // once, without chasing: when a concurrent union got there first, atomicMin returns the parent found
+    // and the loop continues from it.
     __device__ void GlobalUnion(gptr<uint32_t> L, uint32_t a, uint32_t b, const uint32_t n, gptr<uint32_t> err)
     {
+        if (a < n && b < n && a != b)
+        {
+            const uint32_t lo = min(a, b), hi = max(a, b);
+            const uint32_t old = __hip_atomic_fetch_min(&L[hi], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old == hi || old == lo)
+                return;
+            a = lo;
+            b = old;
+        }
         for (;;)
         {
             if (a >= n || b >= n)
