@@ -18,9 +18,10 @@ Two more workloads are reported in the same JSON line:
            [1e3, 1e5]; a tenth of them warmup) at an offered load of 0.8 of the plain 4096^2 rate, through the
            continuous-batching server; p50/p99 completion latency and aggregate spin-updates/s.
 
-Roofline: the dominant kernel (bond activation + tile labelling) is HBM-bound; its algorithmic bytes
-are 13 per site per launch (spin read 1, lcg32 state read+write 8, label write 4). The whole sweep moves
-19 bytes per site (plus the flip: label read 4, spin read+write 2).
+Roofline: the dominant kernel (bond activation + tile labelling, which also applies the previous sweep's
+cluster flips as it loads the spins) is HBM-bound; its algorithmic bytes are 18 per site per launch (spin
+read 1 and write 1, lcg32 state read+write 8, label read 4 (the previous sweep's) and write 4). The whole
+sweep's figure is 19 bytes per site (bond phase 13, flip phase: label read 4, spin read+write 2).
 """
 from __future__ import annotations
 
@@ -37,8 +38,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E
-BOND_LABEL_BYTES_PER_SITE = 13  # spin 1 + lcg32 r/w 8 + label w 4
-SWEEP_BYTES_PER_SITE = 19       # + label r 4 + spin r/w 2
+BOND_LABEL_BYTES_PER_SITE = 18  # spin r/w 2 + lcg32 r/w 8 + label r/w 8 (the flip is applied on load)
+SWEEP_BYTES_PER_SITE = 19       # bond phase 13 + flip phase: label r 4 + spin r/w 2
 T_C = 2.2691853
 
 
@@ -288,7 +289,7 @@ def main() -> int:
                      "avg_launch_ms": k_ms[0]},
         "sweep_roofline": {"bytes_per_site": SWEEP_BYTES_PER_SITE, "achieved_GBps": sweep_gbps,
                            "frac": sweep_gbps / HBM_PEAK_GBPS},
-        "kernels_ms": {"k_bond_label": k_ms[0], "k_merge": k_ms[1], "k_flip": k_ms[2]},
+        "kernels_ms": {"k_bond_label": k_ms[0], "k_merge": k_ms[1], "k_resolve": k_ms[2]},
         "device": info["name"] or info["arch"],
         "copy_calibration": None if copy_ms is None else {"bytes": args.copy_calibration, "ms": copy_ms,
                                                           "GBps": 2 * args.copy_calibration / copy_ms / 1e6},
