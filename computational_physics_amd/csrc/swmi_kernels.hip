@@ -913,7 +913,7 @@ namespace swmi
     // bit `lane` of `bits`: a bond from the site labelled va (this block) to the site labelled vb (block
     // at base_b)
     __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t va,
-                                              const uint32_t vb, const uint32_t base_b, gptr<uint32_t> err)
+                                              const uint32_t vb, const uint32_t base_b, gptr<uint32_t> err, uint64_t* H)
     {
         const uint32_t lane = threadIdx.x & 63u;
         const bool bit = LaneBit(bits);
@@ -925,18 +925,36 @@ namespace swmi
             a = SiteOf(bl.base, bl.stride, va & 0x7FFFu);
             bq = SiteOf(base_b, bl.stride, vb & 0x7FFFu);
         }
-        // one union per distinct pair of roots: the lowest lane holding a pair unions it for all lanes
-        // holding the same pair (below T_c most bonds of an edge join the same two clusters)
+        // One union per distinct pair of roots (below T_c most bonds of an edge join the same two clusters):
+        // the lanes insert their pairs into the wave's LDS hash set H (128 slots for at most 64 pairs, linear
+        // probing, so every insert ends), and the lane whose insert lands leads the union for its pair.
         const uint32_t lo = min(a, bq), hi = max(a, bq);
         const bool want = bit && a != bq;
-        uint64_t todo = __ballot(want);
         bool leader = false;
-        while (todo)
+        if (__ballot(want))
         {
-            const uint32_t l = static_cast<uint32_t>(__builtin_ctzll(todo));
-            const uint32_t llo = ReadLane(lo, l), lhi = ReadLane(hi, l);
-            todo &= ~__ballot(want && lo == llo && hi == lhi);
-            leader = leader || lane == l;
+            H[lane] = 0ull;
+            H[64u + lane] = 0ull;
+            const uint64_t key = (static_cast<uint64_t>(hi) << 32) | lo;   // hi > lo: never 0 (the empty slot)
+            uint32_t slot = Fmix32(lo ^ (hi * 0x9E3779B9u)) & 127u;
+            bool open = want;
+            while (__ballot(open))
+            {
+                if (open)
+                {
+                    uint64_t found = 0ull;
+                    if (__hip_atomic_compare_exchange_strong(&H[slot], &found, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+                    {
+                        leader = true;
+                        open = false;
+                    }
+                    else if (found == key)
+                        open = false;
+                    else
+                        slot = (slot + 1u) & 127u;
+                }
+            }
         }
         if (leader)
             GlobalUnion(Glob(d.parent), lo, hi, d.n0 * d.n1, err);
@@ -957,6 +975,7 @@ namespace swmi
         const uint32_t w = Uniform(threadIdx.x >> 6);
         const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw, twr = d.twr;
         const gptr<uint32_t> err = Glob(d.state) + kStErr;
+        __shared__ uint64_t HS[2][128];   // per wave: hash set of the root pairs of one edge pass
 
         if (w == 0)
         {
@@ -978,7 +997,7 @@ namespace swmi
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
                 const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
                 if (bits)
-                    MergeEdge(d, bl, bits, va, vb, base_r, err);
+                    MergeEdge(d, bl, bits, va, vb, base_r, err, HS[0]);
             }
         }
         else
@@ -992,7 +1011,7 @@ namespace swmi
             const uint32_t va = loc[y * n0 + x], vb = loc[(wrap ? 0u : y + 1u) * n0 + x];
             const uint64_t down_bits = bb[0];
             if (down_bits)
-                MergeEdge(d, bl, down_bits, va, vb, BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u), err);
+                MergeEdge(d, bl, down_bits, va, vb, BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u), err, HS[1]);
         }
     }
 
