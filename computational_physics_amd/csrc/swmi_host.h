@@ -4,7 +4,7 @@
 // An engine owns a set of lattices, their device buffers, one HIP stream and the block work lists over
 // them. Lattices are grouped into kernel classes by the k_bond_label specialisation they need (64-wide
 // tiles or not, tile-height bound 8/16/32/64); a sweep is, for every class, one k_bond_label, one k_merge
-// and one k_flip launch over the class's work list, so a batch of lattices of any sizes costs a handful
+// and one k_resolve launch over the class's work list, so a batch of lattices of any sizes costs a handful
 // of launches per sweep. Repeated launch sequences are captured once as hipGraphs and replayed.
 #pragma once
 

@@ -1,21 +1,23 @@
 // swmi_kernels.hip -- gfx950 kernels of the Swendsen-Wang engine.
 //
 // One sweep = three launches over the engine's block work list (all lattices of the engine at once):
-//   k_bond_label  per block (one workgroup, 4 wavefronts, one tile each): two lcg32 updates per site,
-//                 bond activation against the integer threshold on 64-bit ballot masks (scalar unit),
-//                 row runs from the masks, unions in an LDS union-find (atomicMin keeps parent <= child,
-//                 so roots are cluster minima), block-local uint16 labels with an edge flag, and the
-//                 block-edge bond bits.
+//   k_bond_label  per block (one workgroup, 4 wavefronts, one tile each): applies the previous sweep's
+//                 cluster flip to each site as it loads it (label -> bit of the block's flip bitmap) and
+//                 stores the flipped spin; two lcg32 updates per site; bond activation against the integer
+//                 threshold on 64-bit ballot masks (scalar unit); row runs from the masks, unions in an LDS
+//                 union-find (atomicMin keeps parent <= child, so roots are cluster minima); block-local
+//                 uint16 labels with an edge flag; the block-edge bond bits and the block's halo spins.
+//   k_merge       per block (two waves): unions across block edges by atomicMin on the uint32 parent
+//                 array in HBM, over the roots of edge-reaching clusters only.
+//   k_resolve     per block (two waves): the listed edge-cluster roots are chased in the global parent
+//                 array (one chase per root); the block's flip bitmap (the flip rule applied to every
+//                 cluster's canonical root) is written out and the block's halo spins are flipped in place.
+//                 Advances the lcg32 iteration count; the spin array waits for the next k_bond_label.
+// k_flip applies a pending flip to the spin array; it runs only before something reads the spins.
 // Spin bytes: bit 0 is the spin, bits 3-5 hold the site's current lcg32 parameter selector, i.e. the
 // index k of the Numerical Recipes pair (a_k, c_k) that lane i of its group uses after the group's
 // rotations (sel0[(i + rot) & 63]). The byte is read every sweep anyway, so the generator needs no other
 // per-group data except on the sweeps that rotate (one in 7.5).
-//   k_merge       per block: unions across block edges by atomicMin on the uint32 parent array in HBM,
-//                 over the roots of edge-reaching clusters only.
-//   k_flip        per block (two waves): the listed edge-cluster roots are chased in the global parent
-//                 array (one chase per root) and their flip bits kept in an LDS bitmap; then every site
-//                 flips by its edge root's bit or by its block-local root's parity (spin ^= label & 1).
-//                 Advances the lcg32 iteration count.
 // Reference behaviour: ising/src/swendsen_wang_amd_gpu.cpp:78-467 (four kernels per sweep) and the CPU
 // labeller ising/src/swendsen_wang_cpu.cpp:13-205; the kernels here are written for wave64 CDNA4.
 #include <hip/hip_runtime.h>
