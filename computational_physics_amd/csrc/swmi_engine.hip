@@ -296,6 +296,7 @@ LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
     return d;
 }
 
+// Callers make sure no replay of these execs is in flight (the stream has been synchronised).
 void swmi_engine::DropGraphs()
 {
     for (auto& g : graphs)
@@ -447,7 +448,12 @@ void swmi_engine::Run(uint32_t n, uint32_t sep, int32_t sample_mode)
         if (!entry)
         {
             if (graphs.size() >= kMaxGraphs)
+            {
+                // earlier replays of these execs may still run on the stream (the side-stream branches
+                // are joined into it inside each graph): destroy them only once they have finished
+                Check(hipStreamSynchronize(stream), "graph eviction");
                 DropGraphs();
+            }
             hipGraph_t graph = nullptr;
             Check(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture");
             try

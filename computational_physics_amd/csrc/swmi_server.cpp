@@ -4,8 +4,8 @@
 // jobs are sorted by lattice size into lanes; every lane is an engine with its own HIP stream and work
 // list and runs chunks of `chunk_sweeps` sweeps over all of its resident jobs, so small lattices are not
 // paced by large ones and the lanes' kernels overlap on the device. Between two chunks of a lane the
-// worker retires the jobs that finished (each job stops at exactly warmup + sweeps sweeps through its
-// on-device schedule, whatever the chunk boundaries), admits queued jobs while the device-memory budget
+// worker retires the jobs that finished (each job stops at exactly warmup + ceil(sweeps / n_sep) * n_sep
+// sweeps through its on-device schedule, whatever the chunk boundaries), admits queued jobs while the device-memory budget
 // allows, and rebuilds the lane's work list. Device buffers of retired jobs are pooled by shape and
 // reused. A job's result does not depend on the other jobs: spins, random streams and moments are per
 // lattice and the moments are accumulated in the same order as a run alone.
@@ -32,6 +32,9 @@ using namespace swmi;
 
 namespace
 {
+    // measured sweeps of a job: whole groups of n_sep sweeps covering `sweeps`
+    uint32_t MeasuredSweeps(const uint32_t sweeps, const uint32_t n_sep) { return ((sweeps + n_sep - 1) / n_sep) * n_sep; }
+
     struct PendingJob
     {
         uint64_t id = 0;
@@ -169,13 +172,16 @@ bool swmi_server::Admit(Lane& lane)
             stats.admitted += 1;
         }
         l->Initialize(c, lane.eng->stream);
-        LaunchSetSchedule(l->state.ptr, p.job.warmup + p.job.sweeps, p.job.warmup, cfg.n_sep, lane.eng->stream);
+        // the measured sweeps run in whole groups of n_sep, as the reference's loop `for (i = 0; i < n_sweeps;
+        // i += N_Sep)` does (ising/src/ising_2d.cpp:132): ceil(sweeps / n_sep) groups, a sample after each
+        const uint32_t measured = MeasuredSweeps(p.job.sweeps, cfg.n_sep);
+        LaunchSetSchedule(l->state.ptr, p.job.warmup + measured, p.job.warmup, cfg.n_sep, lane.eng->stream);
         Check(hipGetLastError(), "job initialisation");
         ActiveJob a;
         a.p = p;
         a.lat = l.get();
         a.bytes = l->Bytes();
-        a.total = p.job.warmup + p.job.sweeps;
+        a.total = p.job.warmup + measured;
         a.t_admit = Now();
         active_bytes += a.bytes;
         lane.eng->lattices.push_back(std::move(l));
@@ -444,7 +450,8 @@ int swmi_server_submit(swmi_server* server, const swmi_job* job, uint64_t* job_i
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         const swmi_lattice_config c = swmi_server::ConfigOf(*job);
         ValidateConfig(c);
-        if (job->sweeps == 0 || job->warmup % server->cfg.n_sep != 0 || static_cast<uint64_t>(job->warmup) + job->sweeps > (1u << 30))
+        if (job->sweeps == 0 || job->sweeps > (1u << 30) || job->warmup % server->cfg.n_sep != 0 ||
+            static_cast<uint64_t>(job->warmup) + MeasuredSweeps(job->sweeps, server->cfg.n_sep) > (1u << 30))
             throw SwmiError(SWMI_ERR_INVALID, "job needs sweeps > 0, warmup a multiple of n_sep and warmup + sweeps <= 2^30");
         if (Lattice::ResidentBytes(c) > server->cfg.memory_budget)
             throw SwmiError(SWMI_ERR_MEMORY, "job larger than the server's memory budget");

@@ -112,7 +112,10 @@ namespace
         auto& gpu = static_cast<MI355X&>(*device);
 
         Lattice<2> lattice({o.n0, o.n1}, o.seed, o.tile_rows, o.flip_rule == "random" ? SWMI_FLIP_RANDOM : SWMI_FLIP_PARITY);
-        std::cout << "Target name: MI355X (" << gpu.Info().arch << ", " << gpu.Concurrency() << " CUs)" << std::endl;
+        // ising_2d.cpp:101 prints "AMD GPU" for every target that is not the CPU; the device details go to
+        // stderr so that stdout holds the reference's report lines only
+        std::cout << "Target name: " << "AMD GPU" << std::endl;
+        std::clog << "device: MI355X (" << gpu.Info().arch << ", " << gpu.Concurrency() << " CUs)" << std::endl;
         std::cout << "WARMUP: " << o.n_warmup << " sweeps" << std::endl;
         std::cout << "MEASUREMENT: " << o.n_sweeps << " sweeps" << std::endl;
 
@@ -120,7 +123,7 @@ namespace
             lattice.Update<SwendsenWang_2D, LCG32>(o.temperature, gpu);
         lattice.Synchronize();
 
-        double energy = 0.0, energy_squared = 0.0, magnetization = 0.0, samples = 0.0;
+        double energy = 0.0, energy_squared = 0.0, magnetization = 0.0;
         const auto t0 = std::chrono::high_resolution_clock::now();
         if (o.measure == "host")
         {
@@ -132,13 +135,11 @@ namespace
                 energy += e;
                 energy_squared += e * e;
                 magnetization += std::abs(m);
-                samples += 1.0;
             }
         }
         else
         {
             const auto mom = lattice.Measure(o.temperature, o.n_sweeps, kSep);
-            samples = mom[0];
             energy = mom[1];
             energy_squared = mom[2];
             magnetization = mom[3];
@@ -147,12 +148,13 @@ namespace
         const auto t1 = std::chrono::high_resolution_clock::now();
         const double elapsed_s = std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count() * 1.0E-6;
 
-        // report lines of ising_2d.cpp:150-161
+        // report lines of ising_2d.cpp:150-161, with the stream's default precision. As there, the sums are
+        // divided by n_sweeps / N_Sep (integer division), the number of groups when N_Sep divides n_sweeps.
         std::cout << "Update time per spin: "
-                  << elapsed_s * 1.0E9 / (static_cast<double>(o.n_sweeps) * o.n0 * o.n1) << " ns" << std::endl;
-        const double heat_capacity = ((energy_squared / samples) - (energy / samples) * (energy / samples)) *
-                                     static_cast<double>(lattice.NumSites()) / (o.temperature * o.temperature);
-        std::cout.precision(15);
+                  << elapsed_s * 1.0E9 / (static_cast<std::int64_t>(o.n_sweeps) * o.n0 * o.n1) << " ns" << std::endl;
+        const auto samples = o.n_sweeps / kSep;
+        const auto heat_capacity = ((energy_squared / samples) - (energy / samples) * (energy / samples)) *
+                                   lattice.NumSites() / (o.temperature * o.temperature);
         std::cout << "Internal energy per spin: " << energy / samples << std::endl;
         std::cout << "Heat capacity per spin: " << heat_capacity << std::endl;
         std::cout << "Absolute Magnetization per spin: " << magnetization / samples << std::endl;

@@ -191,10 +191,13 @@ int swmi_copy_calibration(size_t bytes, uint32_t repeats, float* ms);
 /* ---- serving: continuous batching of independent jobs. No counterpart in the reference, which runs one
  * lattice per process (ising/include/lattice.hpp:56); each job follows the reference's measurement
  * protocol (ising/src/ising_2d.cpp:122-148): `warmup` sweeps, then `sweeps` sweeps with a sample after
- * every n_sep-th, FP64 moments as swmi_get_moments. A worker thread owns the GPU: jobs are admitted as
- * device memory (memory_budget) and slots (max_jobs) free up, run in size-class lanes (one HIP stream
- * each) in chunks of chunk_sweeps sweeps together with the other resident jobs, and retire after
- * exactly warmup + sweeps sweeps. A job's result is the same as when it runs alone. ---- */
+ * every n_sep-th, FP64 moments as swmi_get_moments. As in the reference's loop (`for (i = 0; i < n_sweeps;
+ * i += N_Sep)`, ising_2d.cpp:132), the measured sweeps come in whole groups of n_sep: a job runs
+ * ceil(sweeps / n_sep) groups and takes one sample after each. A worker thread owns the GPU: jobs are
+ * admitted as device memory (memory_budget) and slots (max_jobs) free up, run in size-class lanes (one HIP
+ * stream each) in chunks of chunk_sweeps sweeps together with the other resident jobs, and retire after
+ * exactly warmup + ceil(sweeps / n_sep) * n_sep sweeps. A job's result is the same as when it runs
+ * alone. ---- */
 typedef struct swmi_server swmi_server;
 
 typedef struct
@@ -223,7 +226,7 @@ typedef struct
     uint64_t job_id, tag;
     uint32_t n0, n1;
     float temperature;
-    uint32_t sweeps_done;    /* warmup + sweeps */
+    uint32_t sweeps_done;    /* warmup + ceil(sweeps / n_sep) * n_sep */
     uint32_t error_flags;    /* as swmi_get_error_flags */
     uint32_t status;         /* SWMI_OK, or the code that ended the job */
     double moments[6];       /* as swmi_get_moments, over the samples after the warmup */

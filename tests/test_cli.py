@@ -59,7 +59,11 @@ def test_cli_host_and_device_measurement_agree(swmi):
     host = run(*args, "--measure=host")
     assert dev.returncode == 0, dev.stderr
     assert host.returncode == 0, host.stderr
-    assert "Target name: MI355X" in dev.stdout and "WARMUP: 200 sweeps" in dev.stdout
+    # the reference's report lines (ising_2d.cpp:101,120-121,152-162): a GPU target prints "AMD GPU"
+    lines = dev.stdout.splitlines()
+    assert lines[:3] == ["Target name: AMD GPU", "WARMUP: 200 sweeps", "MEASUREMENT: 2000 sweeps"], lines
+    assert [l.split(":")[0] for l in lines[3:]] == ["Update time per spin", "Internal energy per spin",
+                                                    "Heat capacity per spin", "Absolute Magnetization per spin"]
     rd, rh = report(dev.stdout), report(host.stdout)
     # the same configurations are sampled in the same order; both sums are sequential FP64
     assert rd["e"] == rh["e"] and rd["m"] == rh["m"] and rd["c"] == rh["c"]
@@ -79,11 +83,12 @@ def test_cli_matches_python_engine(swmi):
         e.run(500, 10)
         mom = e.moments()
     assert mom[0] == 50
-    assert rep["e"] == pytest.approx(mom[1] / mom[0], rel=1e-13)
-    assert rep["m"] == pytest.approx(mom[3] / mom[0], rel=1e-13)
+    # printed with the stream's default precision (6 significant digits, as %g)
+    assert rep["e"] == float(f"{mom[1] / 50:.6g}")
+    assert rep["m"] == float(f"{mom[3] / 50:.6g}")
     t = np.float32(2.2691853142130221)
-    c = (mom[2] / mom[0] - (mom[1] / mom[0]) ** 2) * 4096 / float(t * t)
-    assert rep["c"] == pytest.approx(c, rel=1e-10)
+    c = (mom[2] / 50 - (mom[1] / 50) ** 2) * 4096 / float(t * t)
+    assert rep["c"] == pytest.approx(float(f"{c:.6g}"), rel=1e-5)
 
 
 @pytest.mark.gpu
@@ -96,5 +101,5 @@ def test_cli_random_flip_rule_matches_python_engine(swmi):
         e.reset_measurements()
         e.run(300, 10)
         mom = e.moments()
-    assert rep["e"] == pytest.approx(mom[1] / mom[0], rel=1e-13)
-    assert rep["m"] == pytest.approx(mom[3] / mom[0], rel=1e-13)
+    assert rep["e"] == float(f"{mom[1] / 30:.6g}")
+    assert rep["m"] == float(f"{mom[3] / 30:.6g}")

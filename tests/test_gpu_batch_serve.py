@@ -113,6 +113,11 @@ def test_autotune_picks_a_height_and_results_do_not_depend_on_it(swmi):
         np.testing.assert_array_equal(tuned.spins(), other.spins())
 
 
+def _measured(sweeps, n_sep=10):
+    """Measured sweeps of a served job: whole groups of n_sep (ising_2d.cpp:132 loops while i < n_sweeps)."""
+    return -(-sweeps // n_sep) * n_sep
+
+
 def _plain_job_result(swmi, job):
     """The same job through the plain path: warmup, reset, then samples every 10 sweeps."""
     c = _cfg(swmi, job.n0, job.n1, job.temperature, seed=job.seed, spin_seed=job.spin_seed)
@@ -137,7 +142,7 @@ def _serve(swmi, jobs, delays=None, **kw):
     out = {ids[r["job_id"]]: r for r in res}
     for k, r in out.items():
         assert r["status"] == 0 and r["error_flags"] == 0, r
-        assert r["sweeps_done"] == jobs[k].warmup + jobs[k].sweeps
+        assert r["sweeps_done"] == jobs[k].warmup + _measured(jobs[k].sweeps)
         assert r["t_submit"] <= r["t_admit"] <= r["t_finish"]
     return out, st
 
@@ -163,13 +168,12 @@ def test_server_job_alone_equals_in_mix_equals_plain(swmi):
     mom, em = _plain_job_result(swmi, target)
     np.testing.assert_array_equal(a["moments"], mom)
     assert (a["energy"], a["magnetization"]) == em
-    # the other jobs of the mix against the plain path too (sweeps a multiple of 10 there)
-    for k in (0, 1, 4):
-        j = mix_jobs[k]
-        if j.sweeps % 10 == 0:
-            mom, em = _plain_job_result(swmi, j)
-            np.testing.assert_array_equal(mixed[k]["moments"], mom, err_msg=f"job {k}")
-            assert (mixed[k]["energy"], mixed[k]["magnetization"]) == em
+    # the other jobs of the mix against the plain path too, including sweeps that are not a multiple of
+    # n_sep (both run whole groups of 10 sweeps, as the reference's loop does)
+    for k in (0, 1, 3, 4):
+        mom, em = _plain_job_result(swmi, mix_jobs[k])
+        np.testing.assert_array_equal(mixed[k]["moments"], mom, err_msg=f"job {k}")
+        assert (mixed[k]["energy"], mixed[k]["magnetization"]) == em
 
 
 def test_server_admits_under_a_small_memory_budget(swmi):
@@ -263,11 +267,10 @@ def test_server_many_small_jobs_from_two_threads(swmi):
     by_tag = {r["tag"]: r for r in res}
     for k in (0, 37, 199):
         r = by_tag[k]
-        assert r["status"] == 0 and r["sweeps_done"] == jobs[k].warmup + jobs[k].sweeps
-        if jobs[k].sweeps % 10 == 0:
-            mom, em = _plain_job_result(swmi, jobs[k])
-            np.testing.assert_array_equal(r["moments"], mom)
-            assert (r["energy"], r["magnetization"]) == em
+        assert r["status"] == 0 and r["sweeps_done"] == jobs[k].warmup + _measured(jobs[k].sweeps)
+        mom, em = _plain_job_result(swmi, jobs[k])
+        np.testing.assert_array_equal(r["moments"], mom)
+        assert (r["energy"], r["magnetization"]) == em
 
 
 @pytest.mark.parametrize("bad", [dict(n0=63), dict(sweeps=0), dict(warmup=15), dict(temperature=-1.0),
