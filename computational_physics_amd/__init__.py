@@ -125,6 +125,7 @@ def load_library() -> ctypes.CDLL:
         "swmi_restore": (ctypes.c_int, [vp, u32, vp, u64]),
         "swmi_error_bars": (ctypes.c_int, [vp, u32, u32, P(f64)]),
         "swmi_get_tile_rows": (ctypes.c_int, [vp, u32, P(u32)]),
+        "swmi_get_layout": (ctypes.c_int, [vp, u32, P(u32), P(u32)]),
         "swmi_autotune": (ctypes.c_int, [P(_Config), u32, P(u32), P(f32)]),
         "swmi_autotune_batch": (ctypes.c_int, [P(_Config), u32, u32, P(u32), P(f32)]),
         "swmi_set_schedule": (ctypes.c_int, [vp, u32, u32, u32, u32]),
@@ -337,10 +338,16 @@ class Engine:
         _check(self._lib.swmi_download_labels(self._h, i, out.ctypes.data))
         return out
 
+    def layout(self, i: int = 0) -> tuple[int, int]:
+        """(row pitch, lcg32 groups): the lattice is stored with rows padded to `pitch` sites, in pitch*n1/64 groups."""
+        pitch, groups = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self._lib.swmi_get_layout(self._h, i, ctypes.byref(pitch), ctypes.byref(groups)))
+        return pitch.value, groups.value
+
     def rng_state(self, i: int = 0) -> dict:
-        n = self._n(i)
-        g = n // 64
-        states = np.empty(n, dtype=np.uint32)
+        """lcg32 state of every group of the padded layout: states (64 per group), selector planes, rotations."""
+        _, g = self.layout(i)
+        states = np.empty(64 * g, dtype=np.uint32)
         masks = np.empty(3 * g, dtype=np.uint64)
         rot = np.empty(g, dtype=np.uint32)
         it = ctypes.c_uint32()

@@ -72,12 +72,12 @@ namespace swmi
         if (c.n0 < 2 || c.n1 < 2 || (c.n0 % 2) || (c.n1 % 2))
             bad("SwendsenWang_2D implementation requires evenly sized lattice (got " + std::to_string(c.n0) + " x " +
                 std::to_string(c.n1) + ")");
-        if (!((c.n0 % 64 == 0) || (64 % c.n0 == 0)))
-            bad("extent[0] must be a multiple of 64 or a divisor of 64 (got " + std::to_string(c.n0) + ")");
-        if (c.n0 < 64 && (c.n1 % (64 / c.n0)))
-            bad("n0*n1 must be a multiple of 64 (got " + std::to_string(c.n0) + " x " + std::to_string(c.n1) + ")");
-        if (static_cast<uint64_t>(c.n0) * c.n1 >= (1ull << 31))
-            bad("lattice too large: n0*n1 must be < 2^31");
+        if (static_cast<uint64_t>(PitchOf(c.n0, c.n1)) * c.n1 >= (1ull << 31))
+            bad("lattice too large: n0*n1 (with rows padded to " + std::to_string(PitchOf(c.n0, c.n1)) + " sites) must be < 2^31");
+        // the kernels address a block's part of the per-site arrays through 32-bit buffer views
+        const Geometry g = GeometryOf(c.n0, c.n1, 1);
+        if (BlockSpanBytes(g) >= (1ull << 31))
+            bad("lattice rows too long: " + std::to_string(c.n0) + " sites per row exceed the kernels' 32-bit block views");
         if (!(c.temperature > 0.0f) || !std::isfinite(c.temperature))
             bad("temperature must be positive and finite");
         if (c.flip_rule != SWMI_FLIP_PARITY && c.flip_rule != SWMI_FLIP_RANDOM)
@@ -112,18 +112,49 @@ namespace swmi
         return twr <= 8 ? 8u : (twr <= 16 ? 16u : (twr <= 32 ? 32u : 64u));
     }
 
+    uint32_t PitchOf(const uint32_t n0, const uint32_t n1)
+    {
+        if (n0 % 64u == 0u)
+            return n0;
+        if (n0 > 64u)
+            return (n0 + 63u) / 64u * 64u;
+        for (uint32_t p = 2u; p < 64u; p *= 2u)
+            if (p >= n0 && n1 % (64u / p) == 0u)
+                return p;
+        return 64u;
+    }
+
+    // Bytes of a block's part of the lcg32 state array (from its first to its last site), the largest
+    // buffer view a kernel builds.
+    uint64_t BlockSpanBytes(const Geometry& g)
+    {
+        const uint64_t rows = std::min<uint64_t>(static_cast<uint64_t>(kWavesPerBlock) * g.twr, g.nwr_y);
+        return ((rows - 1u) * 64ull * g.tiles_x + 64u) * 4u;
+    }
+
     Geometry GeometryOf(const uint32_t n0, const uint32_t n1, const uint32_t tile_rows)
     {
         Geometry g;
-        g.tw = std::min<uint32_t>(n0, 64);
+        g.pitch = PitchOf(n0, n1);
+        g.tw = std::min<uint32_t>(g.pitch, 64);
         g.rpw = 64 / g.tw;
-        g.tiles_x = n0 / g.tw;
+        g.tiles_x = g.pitch / g.tw;
         g.nwr_y = n1 / g.rpw;
+        // last tile column: lanes lx < n0 - (tiles_x - 1) * tw of every lattice row hold sites
+        const uint32_t lastw = n0 - (g.tiles_x - 1u) * g.tw;
+        g.elast = lastw - 1u;
+        const uint64_t row_mask = lastw == 64u ? ~0ull : ((1ull << lastw) - 1ull);
+        g.vlast = 0;
+        for (uint32_t j = 0; j < g.rpw; ++j)
+            g.vlast |= row_mask << (j * g.tw);
         uint32_t twr = tile_rows ? tile_rows : DefaultTileRows(n0, n1);
         twr = std::min<uint32_t>(twr, 64u);
         twr = std::min<uint32_t>(twr, g.tw);      // lattice rows per tile <= 64 (right-edge bit mask)
         twr = std::min<uint32_t>(twr, g.nwr_y);
         g.twr = std::max<uint32_t>(twr, 1u);
+        // blocks of very long rows: lower tiles until a block's buffer views fit 31 bits
+        while (g.twr > 1u && BlockSpanBytes(g) >= (1ull << 31))
+            g.twr /= 2u;
         g.blocks_y = (g.nwr_y + kWavesPerBlock * g.twr - 1) / (kWavesPerBlock * g.twr);
         g.n_blocks = g.tiles_x * g.blocks_y;
         // distinct clusters reaching a block's boundary <= boundary sites (2 columns + 2 rows)
@@ -133,10 +164,31 @@ namespace swmi
         return g;
     }
 
+    LatDesc GeometryDesc(const Geometry& g, const uint32_t n0, const uint32_t n1)
+    {
+        LatDesc d{};
+        d.edge_cap = g.edge_cap;
+        d.ecol_stride = g.ecol_stride;
+        d.n0 = g.pitch;
+        d.n0v = n0;
+        d.n1 = n1;
+        d.elast = g.elast;
+        d.vlast = g.vlast;
+        d.tw = g.tw;
+        d.rpw = g.rpw;
+        d.twr = g.twr;
+        d.tiles_x = g.tiles_x;
+        d.blocks_y = g.blocks_y;
+        d.nwr_y = g.nwr_y;
+        d.n_blocks = g.n_blocks;
+        d.halo_stride = g.halo_stride;
+        return d;
+    }
+
     uint64_t Lattice::ResidentBytes(const swmi_lattice_config& c)
     {
         const Geometry g = GeometryOf(c.n0, c.n1, c.tile_rows);
-        const uint64_t n = static_cast<uint64_t>(c.n0) * c.n1, groups = n / 64;
+        const uint64_t n = static_cast<uint64_t>(g.pitch) * c.n1, groups = n / 64;
         return n * (1 + 4 + 2 + 4) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
                4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
                static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4) + 4ull * g.ecol_stride * g.n_blocks +
@@ -159,15 +211,16 @@ namespace swmi
     void Lattice::Allocate(const swmi_lattice_config& c)
     {
         cfg = c;
-        n = static_cast<uint64_t>(c.n0) * c.n1;
-        n_groups = static_cast<uint32_t>(n / 64);
         g = GeometryOf(c.n0, c.n1, c.tile_rows);
-        spins.allocate(n);
-        rng.allocate(n);
+        n = static_cast<uint64_t>(c.n0) * c.n1;
+        np = static_cast<uint64_t>(g.pitch) * c.n1;
+        n_groups = static_cast<uint32_t>(np / 64);
+        spins.allocate(np);
+        rng.allocate(np);
         masks.allocate(3ull * n_groups);
         rot.allocate(n_groups);
-        local.allocate(n);
-        parent.allocate(n);
+        local.allocate(np);
+        parent.allocate(np);
         bbits.allocate(static_cast<size_t>(kBbitWords) * g.n_blocks);
         state_words = kLatStateWords + 4u * c.max_samples;
         state.allocate(state_words);
@@ -190,9 +243,11 @@ namespace swmi
         bond_dump.reset();
         Check(hipMemsetAsync(edge_count.ptr, 0, edge_count.bytes(), stream), "edge list init");
         LaunchInitState(state.ptr, state_words, c.max_samples, 0xFFFFFFFFu, 0u, 0u, stream);
-        LaunchHotStart(spins.ptr, n, c.spin_seed, stream);
+        if (np != n)   // padding columns hold spin 0
+            Check(hipMemsetAsync(spins.ptr, 0, np, stream), "spin init");
+        LaunchHotStart(spins.ptr, n, c.spin_seed, cfg.n0, g.pitch, stream);
         LaunchInitRng(spins.ptr, rng.ptr, masks.ptr, rot.ptr, n_groups, c.seed, stream);
-        LaunchIota(parent.ptr, n, stream);
+        LaunchIota(parent.ptr, np, stream);
         Check(hipMemsetAsync(local.ptr, 0, local.bytes(), stream), "label init");
         Check(hipGetLastError(), "initialisation launch");
     }
@@ -260,7 +315,7 @@ uint64_t swmi_engine::TotalSites() const
 
 LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
 {
-    LatDesc d{};
+    LatDesc d = GeometryDesc(l.g, l.cfg.n0, l.cfg.n1);
     d.spins = l.spins.ptr;
     d.rng = l.rng.ptr;
     d.masks = l.masks.ptr;
@@ -274,25 +329,13 @@ LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
     d.partials = l.partials.ptr;
     d.edge_list = l.edge_list.ptr;
     d.edge_count = l.edge_count.ptr;
-    d.edge_cap = l.g.edge_cap;
     d.ecol = l.ecol.ptr;
-    d.ecol_stride = l.g.ecol_stride;
-    d.n0 = l.cfg.n0;
-    d.n1 = l.cfg.n1;
-    d.tw = l.g.tw;
-    d.rpw = l.g.rpw;
-    d.twr = l.g.twr;
-    d.tiles_x = l.g.tiles_x;
-    d.blocks_y = l.g.blocks_y;
-    d.nwr_y = l.g.nwr_y;
     d.threshold = static_cast<uint32_t>(l.threshold);
     d.block_base = block_base;
-    d.n_blocks = l.g.n_blocks;
     d.flip_rule = l.cfg.flip_rule;
     d.seed = l.cfg.seed;
     d.fbits = l.fbits.ptr;
     d.halo = l.halo.ptr;
-    d.halo_stride = l.g.halo_stride;
     return d;
 }
 
@@ -317,13 +360,17 @@ void swmi_engine::Rebuild()
     {
         const Lattice& l = *lattices[i];
         const bool tw64 = l.g.tw == 64;
-        const uint32_t t = TwrClass(l.g.twr);
-        auto it = std::find_if(next.begin(), next.end(), [&](const KernelClass& k) { return k.tw64 == tw64 && k.twr_t == t; });
+        const bool pad = l.np != l.n;
+        // padded lattices have two tile-height classes (16 and 64)
+        const uint32_t t = pad ? (l.g.twr <= 16 ? 16u : 64u) : TwrClass(l.g.twr);
+        auto it = std::find_if(next.begin(), next.end(),
+                               [&](const KernelClass& k) { return k.tw64 == tw64 && k.pad == pad && k.twr_t == t; });
         if (it == next.end())
         {
             next.emplace_back();
             it = next.end() - 1;
             it->tw64 = tw64;
+            it->pad = pad;
             it->twr_t = t;
         }
         d[i] = Desc(l, static_cast<uint32_t>(it->block_lat.size()));
@@ -378,7 +425,7 @@ void swmi_engine::EnqueueSweeps(uint32_t n)
         hipStream_t s = ClassStream(c);
         for (uint32_t i = 0; i < n; ++i)
         {
-            LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, recording, s);
+            LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, k.pad, recording, s);
             LaunchMerge(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
             LaunchResolve(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
         }
@@ -514,6 +561,16 @@ void swmi_engine::MaterializeOne(const Lattice& l)
 
 namespace
 {
+    // Row-major host values of a lattice's sites from its padded per-site array (row pitch g.pitch).
+    template <typename T, typename F>
+    void UnpadRows(const Lattice& l, const T* padded, T* out, F&& f)
+    {
+        const uint64_t n0 = l.cfg.n0, pitch = l.g.pitch;
+        for (uint64_t y = 0; y < l.cfg.n1; ++y)
+            for (uint64_t x = 0; x < n0; ++x)
+                out[y * n0 + x] = f(padded[y * pitch + x]);
+    }
+
     swmi_engine* Eng(swmi_engine* e)
     {
         if (!e)
@@ -625,7 +682,7 @@ int swmi_upload_spins(swmi_engine* engine, uint32_t lattice, const uint8_t* spin
         DeviceBuffer<uint8_t> tmp;
         tmp.allocate(l.n);
         Check(hipMemcpyAsync(tmp.ptr, spins, l.n, hipMemcpyHostToDevice, engine->stream), "spin upload");
-        LaunchSetSpins(l.spins.ptr, tmp.ptr, l.n, engine->stream);
+        LaunchSetSpins(l.spins.ptr, tmp.ptr, l.n, l.cfg.n0, l.g.pitch, engine->stream);
         LaunchSetApplied(l.state.ptr, engine->stream);   // the uploaded spins are the chain's state
         engine->LaunchCheck("spin upload");
         Check(hipStreamSynchronize(engine->stream), "spin upload");
@@ -639,10 +696,10 @@ int swmi_download_spins(swmi_engine* engine, uint32_t lattice, uint8_t* spins)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = Eng(engine)->At(lattice);
         engine->MaterializeOne(l);
-        Check(hipMemcpyAsync(spins, l.spins.ptr, l.n, hipMemcpyDeviceToHost, engine->stream), "spin download");
+        std::vector<uint8_t> buf(l.np);
+        Check(hipMemcpyAsync(buf.data(), l.spins.ptr, l.np, hipMemcpyDeviceToHost, engine->stream), "spin download");
         Check(hipStreamSynchronize(engine->stream), "spin download");
-        for (uint64_t i = 0; i < l.n; ++i)   // bits 3-5 carry the lcg32 selectors
-            spins[i] &= 1u;
+        UnpadRows(l, buf.data(), spins, [](uint8_t v) { return static_cast<uint8_t>(v & 1u); });   // bits 3-5: selectors
     });
 }
 
@@ -673,6 +730,21 @@ int swmi_get_threshold(const swmi_engine* engine, uint32_t lattice, float* p_add
             *p_add = l.p_add;
         if (threshold)
             *threshold = l.threshold;
+    });
+}
+
+int swmi_get_layout(const swmi_engine* engine, uint32_t lattice, uint32_t* pitch, uint32_t* n_groups)
+{
+    return Guard([&] {
+        if (!engine)
+            throw SwmiError(SWMI_ERR_INVALID, "null engine");
+        if (lattice >= engine->lattices.size())
+            throw SwmiError(SWMI_ERR_INVALID, "lattice index out of range");
+        const Lattice& l = *engine->lattices[lattice];
+        if (pitch)
+            *pitch = l.g.pitch;
+        if (n_groups)
+            *n_groups = l.n_groups;
     });
 }
 
@@ -836,7 +908,7 @@ int swmi_record_bonds(swmi_engine* engine, int enable)
             if (!l)
                 continue;
             if (enable && !l->bond_dump.ptr)
-                l->bond_dump.allocate(l->n);
+                l->bond_dump.allocate(l->np);
             if (!enable)
                 l->bond_dump.reset();
         }
@@ -854,8 +926,10 @@ int swmi_download_bonds(swmi_engine* engine, uint32_t lattice, uint8_t* bonds)
         Lattice& l = Eng(engine)->At(lattice);
         if (!l.bond_dump.ptr)
             throw SwmiError(SWMI_ERR_STATE, "bond recording is not enabled");
-        Check(hipMemcpyAsync(bonds, l.bond_dump.ptr, l.n, hipMemcpyDeviceToHost, engine->stream), "bond download");
+        std::vector<uint8_t> buf(l.np);
+        Check(hipMemcpyAsync(buf.data(), l.bond_dump.ptr, l.np, hipMemcpyDeviceToHost, engine->stream), "bond download");
         Check(hipStreamSynchronize(engine->stream), "bond download");
+        UnpadRows(l, buf.data(), bonds, [](uint8_t v) { return v; });
     });
 }
 
@@ -865,12 +939,25 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
         if (!bonds)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = Eng(engine)->At(lattice);
+        for (uint64_t i = 0; i < l.n; ++i)
+            if (bonds[i] > 3)
+                throw SwmiError(SWMI_ERR_INVALID, "bond bytes must be 0..3");
         if (!l.bond_in.ptr)
-            l.bond_in.allocate(l.n);
-        Check(hipMemcpyAsync(l.bond_in.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
+            l.bond_in.allocate(l.np);
+        if (l.np == l.n)
+            Check(hipMemcpyAsync(l.bond_in.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
+        else
+        {
+            DeviceBuffer<uint8_t> tmp;
+            tmp.allocate(l.n);
+            Check(hipMemcpyAsync(tmp.ptr, bonds, l.n, hipMemcpyHostToDevice, engine->stream), "bond upload");
+            LaunchPadBytes(l.bond_in.ptr, tmp.ptr, l.np, l.cfg.n0, l.g.pitch, engine->stream);
+            Check(hipStreamSynchronize(engine->stream), "bond upload");
+        }
         engine->MaterializeOne(l);   // the injected pass replaces the labels a pending flip needs
         engine->OneLattice(l, l.bond_in.ptr);
-        LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, true, engine->stream);
+        LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, l.np != l.n, true,
+                        engine->stream);
         LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, engine->stream);
         LaunchWriteLabels(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("label launch");
@@ -887,7 +974,17 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
         engine->OneLattice(l, nullptr);
         LaunchWriteLabels(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("resolve launch");
-        Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
+        if (l.np == l.n)
+            Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
+        else
+        {
+            DeviceBuffer<uint32_t> tmp;
+            tmp.allocate(l.n);
+            LaunchUnpadLabels(tmp.ptr, l.parent.ptr, l.n, l.cfg.n0, l.g.pitch, engine->stream);
+            engine->LaunchCheck("label unpad launch");
+            Check(hipMemcpyAsync(labels, tmp.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
+            Check(hipStreamSynchronize(engine->stream), "label download");
+        }
         Check(hipStreamSynchronize(engine->stream), "label download");
     });
 }
@@ -897,7 +994,7 @@ int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uin
     return Guard([&] {
         Lattice& l = Eng(engine)->At(lattice);
         if (states)
-            Check(hipMemcpyAsync(states, l.rng.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "rng export");
+            Check(hipMemcpyAsync(states, l.rng.ptr, l.np * 4, hipMemcpyDeviceToHost, engine->stream), "rng export");
         if (masks)
             Check(hipMemcpyAsync(masks, l.masks.ptr, 3ull * l.n_groups * 8, hipMemcpyDeviceToHost, engine->stream), "rng export");
         if (rot)
@@ -917,7 +1014,7 @@ namespace
         uint32_t seed, flip_rule;
         float temperature;
         uint32_t state_words;
-        uint64_t n;              // sites
+        uint64_t n;              // padded sites (row pitch * n1)
         uint32_t n_groups;
         uint32_t reserved[5];
     };
@@ -925,7 +1022,7 @@ namespace
 
     uint64_t CheckpointBytes(const Lattice& l)
     {
-        return sizeof(CheckpointHeader) + l.n * (1 + 4) + l.n_groups * (3ull * 8 + 4) + 4ull * l.state_words;
+        return sizeof(CheckpointHeader) + l.np * (1 + 4) + l.n_groups * (3ull * 8 + 4) + 4ull * l.state_words;
     }
 }
 
@@ -953,7 +1050,7 @@ int swmi_checkpoint(swmi_engine* engine, uint32_t lattice, void* buffer, uint64_
         h.flip_rule = l.cfg.flip_rule;
         h.temperature = l.cfg.temperature;
         h.state_words = l.state_words;
-        h.n = l.n;
+        h.n = l.np;
         h.n_groups = l.n_groups;
         uint8_t* out = static_cast<uint8_t*>(buffer);
         std::memcpy(out, &h, sizeof(h));
@@ -962,8 +1059,8 @@ int swmi_checkpoint(swmi_engine* engine, uint32_t lattice, void* buffer, uint64_
             Check(hipMemcpyAsync(q, src, nb, hipMemcpyDeviceToHost, engine->stream), "checkpoint");
             q += nb;
         };
-        get(l.spins.ptr, l.n);
-        get(l.rng.ptr, 4 * l.n);
+        get(l.spins.ptr, l.np);
+        get(l.rng.ptr, 4 * l.np);
         get(l.masks.ptr, 24ull * l.n_groups);
         get(l.rot.ptr, 4ull * l.n_groups);
         get(l.state.ptr, 4ull * l.state_words);
@@ -981,7 +1078,7 @@ int swmi_restore(swmi_engine* engine, uint32_t lattice, const void* buffer, uint
         std::memcpy(&h, buffer, sizeof(h));
         if (std::memcmp(h.magic, "SWMICKP1", 8) != 0)
             throw SwmiError(SWMI_ERR_INVALID, "not an swmi checkpoint");
-        if (h.n0 != l.cfg.n0 || h.n1 != l.cfg.n1 || h.state_words != l.state_words || h.n != l.n || h.n_groups != l.n_groups)
+        if (h.n0 != l.cfg.n0 || h.n1 != l.cfg.n1 || h.state_words != l.state_words || h.n != l.np || h.n_groups != l.n_groups)
             throw SwmiError(SWMI_ERR_INVALID, "checkpoint of a " + std::to_string(h.n0) + " x " + std::to_string(h.n1) +
                                                   " lattice (series words " + std::to_string(h.state_words) +
                                                   ") does not fit this lattice");
@@ -998,8 +1095,8 @@ int swmi_restore(swmi_engine* engine, uint32_t lattice, const void* buffer, uint
             Check(hipMemcpyAsync(dst, q, nb, hipMemcpyHostToDevice, engine->stream), "restore");
             q += nb;
         };
-        put(l.spins.ptr, l.n);
-        put(l.rng.ptr, 4 * l.n);
+        put(l.spins.ptr, l.np);
+        put(l.rng.ptr, 4 * l.np);
         put(l.masks.ptr, 24ull * l.n_groups);
         put(l.rot.ptr, 4ull * l.n_groups);
         put(l.state.ptr, 4ull * l.state_words);
@@ -1105,7 +1202,8 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
         {
             Check(hipEventRecord(ev[0], engine->stream), "event");
             for (auto& k : engine->classes)
-                LaunchBondLabel(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, engine->recording, engine->stream);
+                LaunchBondLabel(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, k.pad, engine->recording,
+                                engine->stream);
             Check(hipEventRecord(ev[1], engine->stream), "event");
             for (auto& k : engine->classes)
                 LaunchMerge(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, engine->stream);

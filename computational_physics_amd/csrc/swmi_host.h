@@ -105,13 +105,22 @@ namespace swmi
         size_t bytes() const { return count * sizeof(T); }
     };
 
-    // Tile geometry of a lattice (see swmi_internal.h for the meaning of the fields).
+    // Tile geometry of a lattice (see swmi_internal.h for the meaning of the fields). pitch: the padded row
+    // width; elast, vlast: the last tile column's last-site lane and site lanes.
     struct Geometry
     {
-        uint32_t tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0, ecol_stride = 0,
-                 halo_stride = 0;
+        uint32_t pitch = 0, tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0,
+                 ecol_stride = 0, halo_stride = 0, elast = 0;
+        uint64_t vlast = 0;
     };
+    // Row pitch of an n0 x n1 lattice: n0 when the wave64 tiling takes it as it is, else the next width it
+    // takes (the next multiple of 64 above 64; below, the smallest power of two p >= n0 with n1 % (64/p) == 0).
+    uint32_t PitchOf(uint32_t n0, uint32_t n1);
+    // Bytes of a block's part of the lcg32 state array, the largest buffer view the kernels build.
+    uint64_t BlockSpanBytes(const Geometry& g);
     Geometry GeometryOf(uint32_t n0, uint32_t n1, uint32_t tile_rows);
+    // Descriptor with the geometry fields of a lattice filled in (buffers null).
+    LatDesc GeometryDesc(const Geometry& g, uint32_t n0, uint32_t n1);
     // Tile height for tile_rows == 0: the autotuned value for this extent if one was measured in this
     // process, else 16 wave rows.
     uint32_t DefaultTileRows(uint32_t n0, uint32_t n1);
@@ -121,8 +130,9 @@ namespace swmi
     struct Lattice
     {
         swmi_lattice_config cfg{};
-        uint64_t n = 0;
-        uint32_t n_groups = 0;
+        uint64_t n = 0;            // sites, n0 * n1
+        uint64_t np = 0;           // padded sites, pitch * n1 (the per-site arrays hold np entries)
+        uint32_t n_groups = 0;     // np / 64
         Geometry g;
         float p_add = 0.0f;
         uint64_t threshold = 0;
@@ -157,6 +167,7 @@ namespace swmi
     struct KernelClass
     {
         bool tw64 = true;
+        bool pad = false;                    // padded lattices (the PAD specialisation of k_bond_label)
         uint32_t twr_t = 16;                 // k_bond_label TWR_MAX
         uint32_t max_twr = 1;
         std::vector<uint32_t> block_lat;     // block -> lattice slot

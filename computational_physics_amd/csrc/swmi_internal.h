@@ -1,6 +1,13 @@
 // swmi_internal.h -- device data layout shared by the kernels and the engine.
 //
-// HBM layout of one lattice (n = n0*n1 sites, n % 64 == 0, groups of 64 sites):
+// Padding. A lattice of width n0v is stored with a row pitch n0 >= n0v that the wave64 tiling takes: n0v
+// itself when it is a multiple of 64 or a divisor of 64 (with n1 a multiple of 64/n0v), else the next
+// multiple of 64 (n0v > 64) or the smallest power of two p >= n0v with n1 % (64/p) == 0. Site (x, y) is
+// padded site y*n0 + x; the columns x in [n0v, n0) are padding: their lanes draw (they belong to an lcg32
+// group) but bond to nothing, and every kernel that reduces or reports sites skips them. Below, n is the
+// padded site count n0*n1 and "site" means padded site.
+//
+// HBM layout of one lattice (n = n0*n1 padded sites, n % 64 == 0, groups of 64 sites):
 //   spins    uint8[n]        bit 0: spin {0,1}; bits 3-5: the site's current lcg32 parameter selector
 //   rng      uint32[n]       per-site lcg32 state (lane i of group g = site 64g+i)
 //   masks    uint64[3*n/64]  initial (a,c) selector of each lane as three bit planes (read-only)
@@ -70,7 +77,10 @@ namespace swmi
         uint16_t* ecol;          // block edge-column labels: [block][side][lattice row in block]
         uint32_t ecol_stride;    // lattice rows per block: kWavesPerBlock * twr * rpw
         uint32_t edge_cap;       // capacity of one tile's list: min(tile sites, block boundary sites)
-        uint32_t n0, n1;
+        uint32_t n0, n1;         // row pitch (padded width) and height
+        uint32_t n0v;            // lattice width: columns [n0v, n0) are padding
+        uint32_t elast;          // lane-in-tile (lx) of a row's last site in the last tile column
+        uint64_t vlast;          // lanes of a wave row of the last tile column that hold sites
         uint32_t tw;             // tile width in sites: min(n0, 64)
         uint32_t rpw;            // lattice rows per wave row: 64 / tw
         uint32_t twr;            // wave rows per tile
@@ -128,8 +138,9 @@ namespace swmi
     // Launch wrappers (swmi_kernels.hip). Work list: block_lat[b] = lattice of global block b.
     // all_tw64: every lattice of the work list has n0 % 64 == 0 (64-wide tiles); selects the specialised kernel
     // debug: compiles in injected bonds (LatDesc::bond_in) and bond recording (LatDesc::bond_dump)
+    // pad: some lattice of the work list is padded (selects the PAD specialisation, correct for every lattice)
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t max_twr,
-                         bool all_tw64, bool debug, hipStream_t stream);
+                         bool all_tw64, bool pad, bool debug, hipStream_t stream);
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
     // end of a sweep: flip bits and halos of every block, then iteration += 2
     void LaunchResolve(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, hipStream_t stream);
@@ -148,8 +159,13 @@ namespace swmi
     void LaunchSetSchedule(uint32_t* state, uint32_t sweeps, uint32_t warmup, uint32_t n_sep, hipStream_t stream);
     void LaunchInitRng(uint8_t* spins, uint32_t* rng, uint64_t* masks, uint32_t* rot, uint32_t n_groups, uint32_t seed,
                        hipStream_t stream);
-    void LaunchSetSpins(uint8_t* spins, const uint8_t* src, uint64_t n, hipStream_t stream);
-    void LaunchHotStart(uint8_t* spins, uint64_t n, uint32_t spin_seed, hipStream_t stream);
+    // n row-major spins (width n0v) into the padded spin bytes (pitch), keeping the selector bits
+    void LaunchSetSpins(uint8_t* spins, const uint8_t* src, uint64_t n, uint32_t n0v, uint32_t pitch, hipStream_t stream);
+    // row-major bytes (width n0v) to np padded bytes (pitch), padding 0
+    void LaunchPadBytes(uint8_t* dst, const uint8_t* src, uint64_t np, uint32_t n0v, uint32_t pitch, hipStream_t stream);
+    // padded canonical labels to the n row-major labels of the row-major sites
+    void LaunchUnpadLabels(uint32_t* dst, const uint32_t* src, uint64_t n, uint32_t n0v, uint32_t pitch, hipStream_t stream);
+    void LaunchHotStart(uint8_t* spins, uint64_t n, uint32_t spin_seed, uint32_t n0v, uint32_t pitch, hipStream_t stream);
     void LaunchCopy(const uint4* src, uint4* dst, uint64_t n16, hipStream_t stream);
     void LaunchIota(uint32_t* labels, uint64_t n, hipStream_t stream);
 }

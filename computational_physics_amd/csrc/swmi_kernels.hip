@@ -22,6 +22,7 @@
 // labeller ising/src/swendsen_wang_cpu.cpp:13-205; the kernels here are written for wave64 CDNA4.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "swmi_internal.h"
@@ -317,7 +318,10 @@ namespace swmi
     // plus kEdgeFlag) is P[root] >> 2.
     // DBG: the inspection modes (injected bonds instead of generated ones, bond recording) are compiled in;
     // the sweep kernel proper has neither branch.
-    template <uint32_t TWR_MAX, bool TW64, bool DBG>
+    // PAD: the lattice is stored with a row pitch d.n0 above its width d.n0v; in the last tile column only
+    // the lanes up to d.elast hold sites (mask d.vlast), the rest are padding: they bond to nothing, their
+    // draws are unused, and the row's last site bonds across the lattice edge instead of lane tw - 1.
+    template <uint32_t TWR_MAX, bool TW64, bool DBG, bool PAD>
     __device__ __forceinline__ void BondLabel(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                               const uint32_t total_blocks)
     {
@@ -359,7 +363,10 @@ namespace swmi
         const uint32_t lx = lane & (tw - 1u);
         const uint32_t lrow = TW64 ? 0u : lane / tw;
         const uint32_t x = bl.sx * tw + lx;
-        const bool rbound = lx == tw - 1u;
+        const bool lastcol = PAD && bl.sx + 1u == d.tiles_x;
+        const uint64_t vmask = lastcol ? d.vlast : ~0ull;   // lanes that hold sites
+        const uint32_t elane = lastcol ? d.elast : tw - 1u;  // lx of a lattice row's last site in this tile
+        const bool rbound = lx == elane;
         const bool injected = DBG && d.bond_in != nullptr;
         const uint32_t thr = d.threshold;
         // sweep schedule: a lattice past its last sweep is left alone by the whole sweep
@@ -426,7 +433,8 @@ namespace swmi
         if (!TW64)
             for (uint32_t j = tw; j < 64u; j += tw)
                 rowstart |= 1ull << j;
-        const uint64_t in_right = ~(rowstart << (tw - 1u));
+        const uint64_t redge = rowstart << elane;   // the last site of every lattice row of the wave row
+        const uint64_t in_right = PAD ? (vmask & ~redge) : ~redge;
         const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
         const uint64_t lanes_below = (1ull << lane) - 1ull;
 
@@ -575,10 +583,16 @@ namespace swmi
                                 __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(sp[i]), rs_spins, lane, Uniform(wr * bl.stride), 0);
                             const uint64_t U1 = __ballot(u1 < thr);
                             U2 = __ballot(st < thr);
-                            // right neighbour's spin: next lane, or for the last column the halo (64-wide
-                            // tiles) or the first column of the same lattice row (narrow tiles: n0 == tw)
-                            const uint64_t RS = TW64 ? ((S >> 1) | (((HR >> i) & 1ull) << 63))
-                                                     : (((S >> 1) & in_right) | ((S << (tw - 1u)) & ~in_right));
+                            // right neighbour's spin: next lane, or for a row's last site the halo (64-wide
+                            // tiles) or the first column of the same lattice row (narrow tiles: one tile
+                            // column)
+                            uint64_t RS;
+                            if (!PAD)
+                                RS = TW64 ? ((S >> 1) | (((HR >> i) & 1ull) << 63))
+                                          : (((S >> 1) & in_right) | ((S << (tw - 1u)) & ~in_right));
+                            else
+                                RS = TW64 ? (((S >> 1) & ~redge) | (((HR >> i) & 1ull) << elane))
+                                          : (((S >> 1) & ~redge) | ((S << elane) & redge));
                             Ball = ~(S ^ RS) & U1;
                         }
                         else
@@ -586,13 +600,18 @@ namespace swmi
                             Ball = __ballot((aux[i] & 1u) != 0u);
                             U2 = __ballot((aux[i] & 2u) != 0u);
                         }
+                        if (PAD)
+                        {
+                            Ball &= vmask;
+                            U2 &= vmask;
+                        }
                         const uint64_t B = Ball & in_right;
-                        const uint64_t edge = Ball & ~in_right;
+                        const uint64_t edge = Ball & redge;
                         if (TW64)
-                            right_bits |= (edge >> 63) << r;
+                            right_bits |= (PAD ? ((edge >> elane) & 1ull) : (edge >> 63)) << r;
                         else
                             for (uint32_t j = 0; j < rpw; ++j)
-                                right_bits |= ((edge >> (j * tw + tw - 1u)) & 1ull) << (r * rpw + j);
+                                right_bits |= ((edge >> (j * tw + elane)) & 1ull) << (r * rpw + j);
                         if (r == 0)
                             s_first_B[w] = B;
 
@@ -800,9 +819,9 @@ namespace swmi
 
         // sites on the block edge flag their root (lanes off the edge flag their own word of Q). The edge
         // lanes of a row are `side`, plus every lane of the block's first and last lattice rows.
-        const uint64_t side = TW64 ? 0x8000000000000001ull : __ballot(lx == 0 || rbound);
-        const uint64_t top_first = r_begin == 0 ? (TW64 ? ~0ull : __ballot(lrow == 0)) : 0ull;
-        const uint64_t bottom_last = last_wave ? (TW64 ? ~0ull : __ballot(lrow == rpw - 1u)) : 0ull;
+        const uint64_t side = TW64 ? (PAD ? (1ull | (1ull << elane)) : 0x8000000000000001ull) : __ballot(lx == 0 || rbound);
+        const uint64_t top_first = r_begin == 0 ? (TW64 ? vmask : (__ballot(lrow == 0) & vmask)) : 0ull;
+        const uint64_t bottom_last = last_wave ? (TW64 ? vmask : (__ballot(lrow == rpw - 1u) & vmask)) : 0ull;
         // full tiles (the rule) take the row count and the first/last row masks at compile time
         const bool full = TWR_MAX <= kRows && nrow == kRows;
         const auto mark_rows = [&](auto full_t) {
@@ -896,14 +915,14 @@ namespace swmi
     __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bond_label(
         const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat, const uint32_t total_blocks)
     {
-        BondLabel<TWR_MAX, true, DBG>(descs, block_lat, total_blocks);
+        BondLabel<TWR_MAX, true, DBG, false>(descs, block_lat, total_blocks);
     }
 
-    template <uint32_t TWR_MAX, bool TW64, bool DBG>
+    template <uint32_t TWR_MAX, bool TW64, bool DBG, bool PAD>
     __global__ __launch_bounds__(256) void k_bond_label_any(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                             const uint32_t total_blocks)
     {
-        BondLabel<TWR_MAX, TW64, DBG>(descs, block_lat, total_blocks);
+        BondLabel<TWR_MAX, TW64, DBG, PAD>(descs, block_lat, total_blocks);
     }
 
     // =============================================================================================
@@ -1022,22 +1041,31 @@ namespace swmi
     // swendsen_wang_amd_gpu.cpp:403-467), split into the decision (k_resolve, end of every sweep) and its
     // application to the spin array (the next sweep's k_bond_label, or k_flip when something reads spins)
     // =============================================================================================
-    __device__ __forceinline__ uint32_t FlipBit(const uint32_t root, const bool random_rule, const uint32_t key)
+    // Row-major site index of a padded site index p (d.n0 is the row pitch, d.n0v the lattice width).
+    __device__ __forceinline__ uint32_t RealSite(const LatDesc& d, const uint32_t p)
     {
-        return random_rule ? (Fmix32(root ^ key) >> 31) : (root & 1u);
+        return d.n0v == d.n0 ? p : (p / d.n0) * d.n0v + p % d.n0;
+    }
+
+    // The flip rule for the cluster whose canonical root is the padded site `root`. The pitch and the width
+    // are even, so a padded index and its row-major index have the same parity.
+    __device__ __forceinline__ uint32_t FlipBit(const LatDesc& d, const uint32_t root, const bool random_rule, const uint32_t key)
+    {
+        return random_rule ? (Fmix32(RealSite(d, root) ^ key) >> 31) : (root & 1u);
     }
 
     // A block's flip bitmap has one bit per block-local index q: for the root of a cluster that stays
     // inside the block, the rule applied to the root's own site; for an edge-reaching root, the rule
     // applied to its canonical label (set by EdgeRootBits). Word w of the rule part: bit j for q = 32w + j
     // (block bases and row strides are even, so under the parity rule q's parity is its site's parity).
-    __device__ __forceinline__ uint32_t RuleWord(const Block& bl, const uint32_t w, const bool random_rule, const uint32_t key)
+    __device__ __forceinline__ uint32_t RuleWord(const LatDesc& d, const Block& bl, const uint32_t w, const bool random_rule,
+                                                 const uint32_t key)
     {
         if (!random_rule)
             return 0xAAAAAAAAu;
         uint32_t v = 0;
         for (uint32_t j = 0; j < 32u; ++j)
-            v |= FlipBit(SiteOf(bl.base, bl.stride, 32u * w + j), true, key) << j;
+            v |= FlipBit(d, SiteOf(bl.base, bl.stride, 32u * w + j), true, key) << j;
         return v;
     }
 
@@ -1089,8 +1117,8 @@ namespace swmi
                 x = p;
                 p = CachedLoad(&L[x]);
             }
-            const uint32_t own = FlipBit(SiteOf(bl.base, bl.stride, q), random_rule, key);
-            if (FlipBit(x, random_rule, key) != own)
+            const uint32_t own = FlipBit(d, SiteOf(bl.base, bl.stride, q), random_rule, key);
+            if (FlipBit(d, x, random_rule, key) != own)
                 (void)__hip_atomic_fetch_xor(&fb[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
@@ -1146,7 +1174,7 @@ namespace swmi
             }
         }
         for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
-            fb[i] = RuleWord(bl, i, random_rule, key);
+            fb[i] = RuleWord(d, bl, i, random_rule, key);
         __syncthreads();
         EdgeRootBits<kThreads>(d, bl, fb, random_rule, key, qmax, ok);
         __syncthreads();
@@ -1395,17 +1423,21 @@ namespace swmi
         const uint32_t w = Uniform(threadIdx.x >> 6);
         const uint32_t r_begin = w * d.twr;
         const uint32_t nrow = bl.nwr > r_begin ? min(d.twr, bl.nwr - r_begin) : 0u;
-        const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw;
+        const uint32_t pitch = d.n0, n0v = d.n0v, n1 = d.n1, tw = d.tw, rpw = d.rpw;
         const uint32_t x = bl.sx * tw + (lane & (tw - 1u));
+        // padding columns (x >= n0v) hold no site; the right neighbour of column n0v - 1 is column 0
+        const bool valid = x < n0v;
+        const int32_t nv = __popcll(__ballot(valid));
+        const uint32_t xr = (x + 1u >= n0v) ? 0u : x + 1u;
         int32_t e = 0, m = 0;
         for (uint32_t r = 0; r < nrow; ++r)
         {
             const uint32_t y = (bl.wyb + r_begin + r) * rpw + lane / tw;
-            const uint32_t s = d.spins[y * n0 + x] & 1u;
-            const uint32_t sr = d.spins[y * n0 + ((x + 1u == n0) ? 0u : x + 1u)] & 1u;
-            const uint32_t sd = d.spins[((y + 1u == n1) ? 0u : y + 1u) * n0 + x] & 1u;
-            e += 2 * (__popcll(__ballot(s == sr)) + __popcll(__ballot(s == sd))) - 128;
-            m += 2 * __popcll(__ballot(s != 0)) - 64;
+            const uint32_t s = d.spins[y * pitch + x] & 1u;
+            const uint32_t sr = d.spins[y * pitch + xr] & 1u;
+            const uint32_t sd = d.spins[((y + 1u == n1) ? 0u : y + 1u) * pitch + x] & 1u;
+            e += 2 * (__popcll(__ballot(valid && s == sr)) + __popcll(__ballot(valid && s == sd))) - 2 * nv;
+            m += 2 * __popcll(__ballot(valid && s != 0)) - nv;
         }
         if (lane == 0)
         {
@@ -1444,7 +1476,7 @@ namespace swmi
         if (threadIdx.x == 0)
         {
             LatState* st = reinterpret_cast<LatState*>(d.state);
-            const double n = static_cast<double>(d.n0) * static_cast<double>(d.n1);
+            const double n = static_cast<double>(d.n0v) * static_cast<double>(d.n1);
             const double ev = -static_cast<double>(red_e[0]) / n;
             const double mv = static_cast<double>(red_m[0]) / n;
             st->moments[0] += 1.0;
@@ -1549,8 +1581,10 @@ namespace swmi
     }
 
     // Hot start (ising/src/lattice.cpp:24-26): site i gets the (i+1)-th drand48 of srand48(spin_seed);
-    // drand48() < 0.5 <=> bit 47 of the 48-bit state is clear. Each thread jumps ahead to its chunk.
-    __global__ __launch_bounds__(256) void k_hot_start(uint8_t* __restrict__ spins, const uint64_t n, const uint32_t spin_seed)
+    // drand48() < 0.5 <=> bit 47 of the 48-bit state is clear. Each thread jumps ahead to its chunk. Site
+    // i = y*n0v + x is stored at y*pitch + x.
+    __global__ __launch_bounds__(256) void k_hot_start(uint8_t* __restrict__ spins, const uint64_t n, const uint32_t spin_seed,
+                                                       const uint32_t n0v, const uint32_t pitch)
     {
         constexpr uint32_t kChunk = 64;
         const uint64_t first = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kChunk;
@@ -1575,16 +1609,46 @@ namespace swmi
         for (uint64_t i = first; i < end; ++i)
         {
             xs = (0x5DEECE66Dull * xs + 0xBull) & kMask;
-            spins[i] = static_cast<uint8_t>((xs >> 47) & 1ull);
+            const uint64_t at = n0v == pitch ? i : (i / n0v) * pitch + i % n0v;
+            spins[at] = static_cast<uint8_t>((xs >> 47) & 1ull);
         }
     }
 
-    // Upload: replace bit 0 of every spin byte, keep the lcg32 selector bits.
-    __global__ __launch_bounds__(256) void k_set_spins(uint8_t* __restrict__ spins, const uint8_t* __restrict__ src, const uint64_t n)
+    // Upload of n row-major spins: replace bit 0 of every spin byte (site i = y*n0v + x at y*pitch + x), keep
+    // the lcg32 selector bits.
+    __global__ __launch_bounds__(256) void k_set_spins(uint8_t* __restrict__ spins, const uint8_t* __restrict__ src, const uint64_t n,
+                                                       const uint32_t n0v, const uint32_t pitch)
     {
         const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
         if (i < n)
-            spins[i] = static_cast<uint8_t>((spins[i] & 0xFEu) | (src[i] & 1u));
+        {
+            const uint64_t at = n0v == pitch ? i : (i / n0v) * pitch + i % n0v;
+            spins[at] = static_cast<uint8_t>((spins[at] & 0xFEu) | (src[i] & 1u));
+        }
+    }
+
+    // Row-major bytes (n0v wide) to the padded layout (pitch wide, padding columns 0), and back.
+    __global__ __launch_bounds__(256) void k_pad_bytes(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, const uint64_t np,
+                                                       const uint32_t n0v, const uint32_t pitch)
+    {
+        const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (p < np)
+        {
+            const uint64_t x = p % pitch;
+            dst[p] = x < n0v ? src[(p / pitch) * n0v + x] : 0u;
+        }
+    }
+
+    // Canonical padded labels (root's padded index per padded site) to row-major labels of row-major sites.
+    __global__ __launch_bounds__(256) void k_unpad_labels(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, const uint64_t n,
+                                                          const uint32_t n0v, const uint32_t pitch)
+    {
+        const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (i < n)
+        {
+            const uint32_t root = src[(i / n0v) * pitch + i % n0v];
+            dst[i] = (root / pitch) * n0v + root % pitch;
+        }
     }
 
     __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ labels, const uint64_t n)
@@ -1608,34 +1672,41 @@ namespace swmi
 
     template <bool TW64, bool DBG>
     static void LaunchBondLabelT(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks,
-                                 const uint32_t max_twr, hipStream_t stream)
+                                 const uint32_t max_twr, const bool pad, hipStream_t stream)
     {
         const uint32_t g = XcdGrid(total_blocks);
         const auto launch = [&](auto kernel) { kernel<<<g, 256, 0, stream>>>(d_desc, d_block_lat, total_blocks); };
-        if (max_twr <= 8)
-            TW64 ? launch(k_bond_label<8, DBG>) : launch(k_bond_label_any<8, false, DBG>);
+        if (pad)   // padded lattices: two tile-height bounds
+        {
+            if (max_twr <= 16)
+                launch(k_bond_label_any<16, TW64, DBG, true>);
+            else
+                launch(k_bond_label_any<64, TW64, DBG, true>);
+        }
+        else if (max_twr <= 8)
+            TW64 ? launch(k_bond_label<8, DBG>) : launch(k_bond_label_any<8, false, DBG, false>);
         else if (max_twr <= 16)
-            TW64 ? launch(k_bond_label<16, DBG>) : launch(k_bond_label_any<16, false, DBG>);
+            TW64 ? launch(k_bond_label<16, DBG>) : launch(k_bond_label_any<16, false, DBG, false>);
         else if (max_twr <= 32)
-            launch(k_bond_label_any<32, TW64, DBG>);
+            launch(k_bond_label_any<32, TW64, DBG, false>);
         else
-            launch(k_bond_label_any<64, TW64, DBG>);
+            launch(k_bond_label_any<64, TW64, DBG, false>);
     }
 
     void LaunchBondLabel(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t max_twr,
-                         const bool all_tw64, const bool debug, hipStream_t stream)
+                         const bool all_tw64, const bool pad, const bool debug, hipStream_t stream)
     {
         if (debug)
         {
             if (all_tw64)
-                LaunchBondLabelT<true, true>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+                LaunchBondLabelT<true, true>(d_desc, d_block_lat, total_blocks, max_twr, pad, stream);
             else
-                LaunchBondLabelT<false, true>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+                LaunchBondLabelT<false, true>(d_desc, d_block_lat, total_blocks, max_twr, pad, stream);
         }
         else if (all_tw64)
-            LaunchBondLabelT<true, false>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+            LaunchBondLabelT<true, false>(d_desc, d_block_lat, total_blocks, max_twr, pad, stream);
         else
-            LaunchBondLabelT<false, false>(d_desc, d_block_lat, total_blocks, max_twr, stream);
+            LaunchBondLabelT<false, false>(d_desc, d_block_lat, total_blocks, max_twr, pad, stream);
     }
 
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
@@ -1691,15 +1762,29 @@ namespace swmi
         k_init_rng<<<(n_groups + 255) / 256, 256, 0, stream>>>(spins, rng, masks, rot, n_groups, seed);
     }
 
-    void LaunchSetSpins(uint8_t* spins, const uint8_t* src, const uint64_t n, hipStream_t stream)
+    void LaunchSetSpins(uint8_t* spins, const uint8_t* src, const uint64_t n, const uint32_t n0v, const uint32_t pitch,
+                        hipStream_t stream)
     {
-        k_set_spins<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, stream>>>(spins, src, n);
+        k_set_spins<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, stream>>>(spins, src, n, n0v, pitch);
     }
 
-    void LaunchHotStart(uint8_t* spins, const uint64_t n, const uint32_t spin_seed, hipStream_t stream)
+    void LaunchPadBytes(uint8_t* dst, const uint8_t* src, const uint64_t np, const uint32_t n0v, const uint32_t pitch,
+                        hipStream_t stream)
+    {
+        k_pad_bytes<<<static_cast<uint32_t>((np + 255) / 256), 256, 0, stream>>>(dst, src, np, n0v, pitch);
+    }
+
+    void LaunchUnpadLabels(uint32_t* dst, const uint32_t* src, const uint64_t n, const uint32_t n0v, const uint32_t pitch,
+                           hipStream_t stream)
+    {
+        k_unpad_labels<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, stream>>>(dst, src, n, n0v, pitch);
+    }
+
+    void LaunchHotStart(uint8_t* spins, const uint64_t n, const uint32_t spin_seed, const uint32_t n0v, const uint32_t pitch,
+                        hipStream_t stream)
     {
         const uint64_t threads = (n + 63) / 64;
-        k_hot_start<<<static_cast<uint32_t>((threads + 255) / 256), 256, 0, stream>>>(spins, n, spin_seed);
+        k_hot_start<<<static_cast<uint32_t>((threads + 255) / 256), 256, 0, stream>>>(spins, n, spin_seed, n0v, pitch);
     }
 
     void LaunchIota(uint32_t* labels, const uint64_t n, hipStream_t stream)

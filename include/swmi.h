@@ -13,9 +13,14 @@
  *           per-spin values are -E/N and M/N                                 (ising/src/lattice.cpp:29-67)
  *   bonds   p_add = 1.0f - (float)expf(-2.0f / T), bond iff spins equal and r < p_add,
  *           evaluated as u < threshold on the uint32 draw u                  (ising/src/swendsen_wang.cpp:100)
- * lcg32 layout (new, the engine's own): group g = sites [64g, 64g+64) is one LCG32_State<64>
+ * lcg32 layout (new, the engine's own): the lattice is held with its rows padded to a pitch P >= n0
+ * (swmi_get_layout): P = n0 if n0 is a multiple of 64 or a divisor of 64 with n1 a multiple of 64/n0,
+ * else the next multiple of 64 (n0 > 64) or the smallest power of two p >= n0 with n1 % (64/p) == 0.
+ * Site (x, y) is padded site y*P + x. Group g = padded sites [64g, 64g+64) is one LCG32_State<64>
  * (random/src/lcg32.cpp:31-84) seeded with seed*0x9E3779B1 + g + 1; each sweep draws twice per site,
- * the first draw decides the bond to (x+1, y), the second the bond to (x, y+1).
+ * the first draw decides the bond to (x+1, y), the second the bond to (x, y+1). The lanes of padding
+ * columns draw with their group and decide nothing. Every array that crosses the ABI (spins, bonds,
+ * labels) is unpadded row-major; only swmi_rng_export and checkpoints hold the padded layout.
  */
 #ifndef SWMI_H
 #define SWMI_H
@@ -49,8 +54,8 @@ typedef struct swmi_engine swmi_engine;
 
 typedef struct
 {
-    uint32_t n0;          /* extent in x (fast index); even; a multiple of 64 or a divisor of 64 */
-    uint32_t n1;          /* extent in y; even; n0*n1 a multiple of 64 and < 2^31 */
+    uint32_t n0;          /* extent in x (fast index); even (swendsen_wang.cpp:45-48), at least 2 */
+    uint32_t n1;          /* extent in y; even, at least 2; P*n1 < 2^31 for the row pitch P */
     float temperature;    /* T > 0 */
     uint32_t seed;        /* lcg32 group seeds: seed*0x9E3779B1 + g + 1 (seed 0: g + 1) */
     uint32_t spin_seed;   /* hot start: srand48(spin_seed); spin = drand48() < 0.5 ? 0 : 1 (1: reference) */
@@ -107,6 +112,10 @@ int swmi_sweep(swmi_engine* engine, uint32_t n_sweeps);
 int swmi_set_graphs(swmi_engine* engine, int enable);
 /* Tile height (in 64-site wave rows) the lattice runs with. */
 int swmi_get_tile_rows(const swmi_engine* engine, uint32_t lattice, uint32_t* tile_rows);
+/* Row pitch P of the lattice's padded layout and its number of lcg32 groups, P*n1/64 (the sizes of the
+ * swmi_rng_export arrays). The reference stores the lattice unpadded (ising/include/lattice.hpp:24); P = n0
+ * for every extent that needs no padding. */
+int swmi_get_layout(const swmi_engine* engine, uint32_t lattice, uint32_t* pitch, uint32_t* n_groups);
 /* Tile-height autotuner: times plain sweeps of this configuration for each tile height in
  * {1, 2, 4, 8, 16, 32, 64} (clamped to the lattice; ms_per_sweep[i] = -1 for a height clamped to one already
  * measured), returns the fastest and remembers it for this extent, so that later lattices created with
@@ -154,8 +163,8 @@ int swmi_download_bonds(swmi_engine* engine, uint32_t lattice, uint8_t* bonds);
 int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds);
 /* Canonical labels of the last labelling (after a sweep: the clusters that sweep flipped). */
 int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels);
-/* Per-site lcg32 states (n_groups*64), selector bit planes (3*n_groups), rotations (n_groups) and the
- * iteration count of the lattice. */
+/* Per-site lcg32 states (n_groups*64, padded layout), selector bit planes (3*n_groups), rotations (n_groups)
+ * and the iteration count of the lattice (n_groups: swmi_get_layout). */
 int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uint64_t* masks, uint32_t* rot,
                     uint32_t* iteration);
 

@@ -583,6 +583,18 @@ void or_refcpu_sweep(or_refcpu* r, uint8_t* spins, float temperature, uint8_t* b
  * built from LCG32_State<64> of random/src/lcg32.cpp:31-84). A rotation of (a, c) by s followed by a
  * rotation by t is a rotation by s + t, so a group keeps only its initial selectors and an offset.
  * ------------------------------------------------------------------------------------------------ */
+uint32_t or_ps_pitch(uint32_t n0, uint32_t n1)
+{
+    if (n0 % 64U == 0U)
+        return n0;
+    if (n0 > 64U)
+        return (n0 + 63U) / 64U * 64U;
+    for (uint32_t p = 2U; p < 64U; p *= 2U)
+        if (p >= n0 && n1 % (64U / p) == 0U)
+            return p;
+    return 64U;
+}
+
 uint32_t or_group_seed(uint32_t seed, uint32_t g)
 {
     return seed * 0x9E3779B1U + g + 1U; /* seed 0 -> g + 1, the reference's "state i seeded with i+1" */
@@ -669,16 +681,25 @@ void or_ps_sweep_rule(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spi
 {
     const uint32_t key = or_flip_key(seed, *iteration);
     const size_t n = (size_t)n0 * n1;
-    const uint32_t n_groups = (uint32_t)((n + 63) / 64);
-    const size_t npad = (size_t)n_groups * 64;
-    uint32_t* u1 = (uint32_t*)malloc(npad * sizeof(uint32_t));
+    const uint32_t pitch = or_ps_pitch(n0, n1);
+    const size_t npad = (size_t)pitch * n1;
+    const uint32_t n_groups = (uint32_t)(npad / 64);
+    uint32_t* u1 = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* u2 = (uint32_t*)malloc(n * sizeof(uint32_t));
     uint8_t* bonds = (uint8_t*)malloc(n);
     uint32_t* labels = (uint32_t*)malloc(n * sizeof(uint32_t));
+    /* the draws of site (x, y) are lane x % 64 of padded group (y * pitch + x) / 64; lanes of the padding
+     * columns x >= n0 advance with their group and decide nothing */
     or_ps_rng_update(n_groups, states, masks, rot, *iteration);
-    memcpy(u1, states, npad * sizeof(uint32_t));
+    for (uint32_t y = 0; y < n1; ++y)
+        for (uint32_t x = 0; x < n0; ++x)
+            u1[(size_t)y * n0 + x] = states[(size_t)y * pitch + x];
     or_ps_rng_update(n_groups, states, masks, rot, *iteration + 1U);
+    for (uint32_t y = 0; y < n1; ++y)
+        for (uint32_t x = 0; x < n0; ++x)
+            u2[(size_t)y * n0 + x] = states[(size_t)y * pitch + x];
     *iteration += 2U;
-    or_ps_bonds(n0, n1, spins, u1, states, threshold, bonds);
+    or_ps_bonds(n0, n1, spins, u1, u2, threshold, bonds);
     or_label(n0, n1, 16, 8, bonds, labels);
     if (flip_rule == 1U)
         or_flip_random(n, spins, labels, key);
@@ -689,6 +710,7 @@ void or_ps_sweep_rule(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spi
     if (labels_out)
         memcpy(labels_out, labels, n * sizeof(uint32_t));
     free(u1);
+    free(u2);
     free(bonds);
     free(labels);
 }

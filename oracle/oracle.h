@@ -80,10 +80,16 @@ void or_refcpu_measure(or_refcpu* r, uint8_t* spins, float temperature, uint32_t
                        double* out);
 
 /* ---- the per-site lcg32 layout of the MI355X engine ----
- * Group g = sites [64g, 64g+64) in row-major order; lane i of the group = site 64g+i. Each group is one
+ * The lattice is laid out with a row pitch P >= n0 (or_ps_pitch): site (x, y) is padded site y*P + x, and
+ * the columns x in [n0, P) are padding. Group g = padded sites [64g, 64g+64), n_groups = P*n1/64; lane i of
+ * the group = padded site 64g+i, so for P = n0 group g is row-major sites [64g, 64g+64). Each group is one
  * LCG32_State<64> seeded with or_group_seed(seed, g). Its (a,c) arrays are held as the initial
  * parameter selector of every lane (3 bit planes, masks[3g+0..2]) plus a rotation offset rot[g]:
- * a[i] = Parameters[sel0[(i + rot) & 63]].a. The iteration count is the same for every group. */
+ * a[i] = Parameters[sel0[(i + rot) & 63]].a. The iteration count is the same for every group. Padding
+ * lanes advance with their group (the rotation may read their state) and decide no bond. */
+/* P = n0 if n0 % 64 == 0; the next multiple of 64 if n0 > 64; else the smallest power of two p >= n0 with
+ * n1 % (64/p) == 0 (64 at most), so that a 64-site group holds 64/p whole padded rows. */
+uint32_t or_ps_pitch(uint32_t n0, uint32_t n1);
 uint32_t or_group_seed(uint32_t seed, uint32_t g);
 void or_ps_init(uint32_t n_groups, uint32_t seed, uint32_t* states, uint64_t* masks, uint32_t* rot);
 /* One Update() of every group, iteration_before -> iteration_before + 1. */
@@ -91,7 +97,8 @@ void or_ps_rng_update(uint32_t n_groups, uint32_t* states, const uint64_t* masks
 /* Bond field from spins and two draws per site (u1 -> +x, u2 -> +y). */
 void or_ps_bonds(uint32_t n0, uint32_t n1, const uint8_t* spins, const uint32_t* u1, const uint32_t* u2,
                  uint64_t threshold, uint8_t* bonds);
-/* One sweep of the engine's layout: two updates, bonds, labels, flip. *iteration += 2.
+/* One sweep of the engine's layout (states, masks and rot of the P*n1/64 groups; spins, bonds and labels in
+ * the unpadded row-major layout): two updates, bonds, labels, flip. *iteration += 2.
  * flip_rule 0: spin ^= root & 1 (reference). bonds_out / labels_out may be NULL. */
 void or_ps_sweep(uint32_t n0, uint32_t n1, uint64_t threshold, uint8_t* spins, uint32_t* states,
                  const uint64_t* masks, uint32_t* rot, uint32_t* iteration, uint8_t* bonds_out, uint32_t* labels_out);

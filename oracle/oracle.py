@@ -69,6 +69,8 @@ def lib() -> ctypes.CDLL:
     L.or_refcpu_create.restype = vp
     L.or_refcpu_destroy.argtypes = [vp]
     L.or_refcpu_sweep.argtypes = [vp, vp, ctypes.c_float, vp, vp]
+    L.or_ps_pitch.argtypes = [u32, u32]
+    L.or_ps_pitch.restype = u32
     L.or_group_seed.argtypes = [u32, u32]
     L.or_group_seed.restype = u32
     L.or_ps_init.argtypes = [u32, u32, vp, vp, vp]
@@ -100,6 +102,11 @@ def label(n0: int, n1: int, bonds: np.ndarray, tile=(16, 8)) -> np.ndarray:
     out = np.empty(n0 * n1, dtype=np.uint32)
     lib().or_label(n0, n1, tile[0], tile[1], b.ctypes.data, out.ctypes.data)
     return out
+
+
+def pitch(n0: int, n1: int) -> int:
+    """Row pitch of the engine's padded layout."""
+    return int(lib().or_ps_pitch(n0, n1))
 
 
 def observables(n0: int, n1: int, spins: np.ndarray) -> tuple[int, int]:
@@ -162,7 +169,8 @@ class PerSite:
         self.n0, self.n1 = n0, n1
         self.seed, self.flip_rule = seed, flip_rule
         self.n = n0 * n1
-        self.groups = (self.n + 63) // 64
+        self.pitch = pitch(n0, n1)
+        self.groups = self.pitch * n1 // 64
         self.states = np.empty(self.groups * 64, dtype=np.uint32)
         self.masks = np.empty(3 * self.groups, dtype=np.uint64)
         self.rot = np.empty(self.groups, dtype=np.uint32)

@@ -10,6 +10,9 @@
 //   ref_sw run <N0> <N1> <T> <warmup> <sweeps>
 //       the measurement protocol of ising/src/ising_2d.cpp:120-162 (observables every 10 sweeps inside
 //       the timed region) with a configurable warmup, printing the reference report lines.
+//   ref_sw series <N0> <N1> <T> <warmup> <sweeps> <out.bin>
+//       the same protocol, writing the int64 pair (sum s(s_r+s_d), sum s) of every sample to out.bin, so
+//       that error bars can be computed from the reference CPU run (the driver prints means only).
 // NUM_THREADS selects the thread count as in the reference (utility/device/device_cpu.hpp:23).
 #include <chrono>
 #include <cmath>
@@ -101,6 +104,31 @@ int main(int argc, char** argv)
         std::cout << "Threads: " << cpu.Concurrency() << std::endl;
         return 0;
     }
-    std::fprintf(stderr, "usage: ref_sw dump N0 N1 T out.bin k1 [k2 ..] | ref_sw run N0 N1 T warmup sweeps\n");
+    if (argc == 8 && !std::strcmp(argv[1], "series"))
+    {
+        const std::uint32_t n0 = std::atoi(argv[2]), n1 = std::atoi(argv[3]);
+        const float temperature = static_cast<float>(std::atof(argv[4]));
+        const std::uint32_t n_warmup = std::atoi(argv[5]), n_sweeps = std::atoi(argv[6]);
+        constexpr std::uint32_t n_sep = 10; // ising_2d.cpp:27
+        std::FILE* out = std::fopen(argv[7], "wb");
+        if (!out)
+            return 1;
+        Lattice<2> lattice({n0, n1});
+        CPU cpu;
+        for (std::uint32_t i = 0; i < n_warmup; ++i)
+            lattice.Update<SwendsenWang_2D, LCG32>(temperature, cpu);
+        for (std::uint32_t i = 0; i < n_sweeps; i += n_sep)
+        {
+            for (std::uint32_t k = 0; k < n_sep; ++k)
+                lattice.Update<SwendsenWang_2D, LCG32>(temperature, cpu);
+            std::int64_t sums[2];
+            Sums(lattice, sums[0], sums[1]);
+            std::fwrite(sums, sizeof(std::int64_t), 2, out);
+        }
+        std::fclose(out);
+        return 0;
+    }
+    std::fprintf(stderr, "usage: ref_sw dump N0 N1 T out.bin k1 [k2 ..] | ref_sw run N0 N1 T warmup sweeps | "
+                         "ref_sw series N0 N1 T warmup sweeps out.bin\n");
     return 1;
 }

@@ -167,10 +167,11 @@ def test_run_series_and_moments(swmi, oracle):
                                rtol=1e-13)
 
 
-@pytest.mark.parametrize("n,T", [(4096, TC), (4096, 2.0), (4096, 2.6), (8192, 2.0)])
+@pytest.mark.parametrize("n,T", [(4096, TC), (4096, 2.0), (4096, 2.6), (8192, 2.0), (8192, TC), (8192, 2.6)])
 def test_large_lattice_one_sweep_bit_exact(swmi, oracle, n, T):
-    """Full-size lattices: one sweep, bonds + labels + spins bit-exact against the oracle. T = 2.0 has one
-    giant cluster (atomic contention in the merge), T = 2.6 many small ones."""
+    """Full-size lattices (the 4096^2 and 8192^2 configurations at all three temperatures): one sweep, bonds +
+    labels + spins bit-exact against the oracle. T = 2.0 has one giant cluster (atomic contention in the
+    merge), T = 2.6 many small ones."""
     n0 = n1 = n
     print(f"{n}^2 T {T} one sweep", flush=True)
     ref = oracle.PerSite(n0, n1, T, seed=1)
@@ -225,3 +226,20 @@ def test_large_lattice_invariants_over_sweeps(swmi, oracle, n, T, sweeps):
         assert np.all(labels[labels] == labels)
         assert int(spins[0]) == s0
         assert e.energy_magnetization() == oracle.observables(n, n, spins)
+
+
+def test_engine_threshold_equals_the_reference_fixture(swmi):
+    """p_add and the integer threshold the engine derives on this host (glibc expf under the engine's
+    flags) equal the bits of the compiled reference (padd.json) and the exhaustively checked thresholds
+    (thresholds.json), for every fixture temperature."""
+    import json
+    from conftest import GOLDEN
+    fixture = json.loads((GOLDEN / "thresholds.json").read_text())
+    cfgs = [swmi.LatticeConfig(64, 64, float(np.float32(float(t))), seed=1) for t in fixture]
+    with swmi.Engine(cfgs) as e:
+        for i, (t, want) in enumerate(fixture.items()):
+            p, thr = e.threshold(i)
+            assert int(np.array([p], dtype=np.float32).view(np.uint32)[0]) == want["p_add_bits"], t
+            assert thr == want["threshold"], t
+            e.set_temperature(float(np.float32(float(t))) * 1.0, i)
+            assert e.threshold(i)[1] == want["threshold"], t

@@ -101,6 +101,50 @@ def test_per_site_layout_equals_full_lcg32_state(oracle):
     assert oracle.lib().or_group_seed(0, 0) == 1 and oracle.lib().or_group_seed(0, 9) == 10
 
 
+@pytest.mark.parametrize("n0,n1", [(64, 64), (96, 96), (100, 100), (48, 34), (130, 18), (80, 66), (2, 2), (6, 8),
+                                   (16, 6), (32, 32), (62, 2), (4, 16), (8, 24), (1000, 10)])
+def test_pitch_is_the_smallest_width_the_wave64_tiling_takes(oracle, n0, n1):
+    """The padded row pitch P: P >= n0, and either P % 64 == 0 (64-site groups within a row) or P divides
+    64 with n1 a multiple of 64 / P (groups of whole padded rows); no smaller width of that form exists."""
+    def ok(p):
+        return p >= n0 and (p % 64 == 0 or (64 % p == 0 and n1 % (64 // p) == 0))
+    p = oracle.pitch(n0, n1)
+    assert ok(p) and p % 2 == 0
+    assert not any(ok(q) for q in range(n0, p))
+    if n0 % 64 == 0 or (64 % n0 == 0 and n1 % (64 // n0) == 0):
+        assert p == n0   # every extent without padding keeps its layout
+
+
+@pytest.mark.parametrize("n0,n1,t", [(100, 6, 2.2691853), (48, 34, 2.0), (130, 4, 2.6), (6, 8, 2.2691853), (2, 2, 1.5)])
+def test_padded_engine_layout_draws_from_the_padded_groups(oracle, n0, n1, t):
+    """The engine layout's bonds at an extent that needs padding, re-derived independently: every padded
+    group is a full LCG32_State<64> (pinned against the compiled reference), the draws of site (x, y) are
+    lane (y*P + x) % 64 of group (y*P + x) // 64 after the first and second update, and a bond needs equal
+    spins and a draw below the threshold."""
+    seed = 7
+    ps = oracle.PerSite(n0, n1, t, seed=seed)
+    P = ps.pitch
+    spins0 = ps.spins.copy()
+    full = []
+    for g in range(P * n1 // 64):
+        st = oracle.Lcg32()
+        oracle.lib().or_lcg32_init(ctypes.byref(st), 64, oracle.lib().or_group_seed(seed, g))
+        full.append(st)
+    draws = []
+    for _ in range(2):
+        for st in full:
+            oracle.lib().or_lcg32_update(ctypes.byref(st))
+        draws.append(np.concatenate([np.array(st.state[:], dtype=np.uint32) for st in full]).reshape(n1, P)[:, :n0])
+    bonds, labels = ps.sweep(record=True)
+    s = spins0.reshape(n1, n0)
+    thr = oracle.threshold(t)
+    want = ((s == np.roll(s, -1, axis=1)) & (draws[0] < thr)).astype(np.uint8)
+    want |= ((s == np.roll(s, -1, axis=0)) & (draws[1] < thr)).astype(np.uint8) << 1
+    np.testing.assert_array_equal(bonds.reshape(n1, n0), want)
+    np.testing.assert_array_equal(labels, components_min_label(n0, n1, bonds))
+    np.testing.assert_array_equal(ps.states.reshape(-1, 64), np.array([st.state[:] for st in full], dtype=np.uint32))
+
+
 # ------------------------------------------------------------------------------------------------
 # bond probability
 # ------------------------------------------------------------------------------------------------
@@ -301,3 +345,33 @@ def test_reference_readme_sample(oracle):
     s = oracle.hot_start(128 * 128, 1)
     e, c, m = r.measure(s, float(np.float32(2.2691853142130221)), 1000, 100000, 10)
     assert ("%g" % e, "%g" % c, "%g" % m) == ("-1.41921", "2.54411", "0.550833")
+
+
+def test_threshold_fixture_is_the_boundary_of_the_float_compare(oracle):
+    """thresholds.json (written with an exhaustive check over all 2^32 draws) holds, for every p_add of the
+    compiled reference, the first draw u at which 2^-32f * (float)u < p_add fails: (float)u is monotone in u,
+    so the compare holds below t and fails from t on exactly when it holds at t - 1 and fails at t."""
+    fixture = json.loads((GOLDEN / "thresholds.json").read_text())
+    padd = json.loads((GOLDEN / "padd.json").read_text())
+    two32 = np.float32(2.3283064370807974e-10)
+    for t, v in fixture.items():
+        assert v["p_add_bits"] == padd[t]
+        p = np.array([v["p_add_bits"]], dtype=np.uint32).view(np.float32)[0]
+        thr = v["threshold"]
+        assert two32 * np.float32(thr - 1) < p and not (thr < 2 ** 32 and two32 * np.float32(thr) < p), t
+        assert oracle.threshold(float(np.float32(float(t)))) == thr
+
+
+def test_reference_cpu_statistics_fixture_reproduces_the_readme_sample():
+    """ref_cpu_stats.json holds the compiled reference's own 128^2 runs; at T_c it is the README's published
+    run (README.md:66-82: E/N -1.41921, C/N 2.54411, |m| 0.550833 at 6 significant digits)."""
+    ref = json.loads((GOLDEN / "ref_cpu_stats.json").read_text())["values"]
+    tc = ref["2.2691853"]
+    assert (f"{tc['e']:g}", f"{tc['c']:g}", f"{tc['abs_m']:g}") == ("-1.41921", "2.54411", "0.550833")
+    exact = {(v["L"], v["T"]): v for v in json.loads((GOLDEN / "exact_values.json").read_text())["values"]}
+    for t, v in ref.items():
+        assert v["samples"] == 10000 and v["e_err"] > 0 and v["c_err"] > 0 and v["abs_m_err"] > 0
+        # the reference's chain is itself within 3 standard errors of the exact values
+        ex = exact[(128, t)]
+        assert abs(v["e"] - ex["U_per_spin"]) < 3 * v["e_err"], t
+        assert abs(v["c"] - ex["C_per_spin"]) < 3 * v["c_err"], t
