@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 
 __all__ = ["SwmiError", "LatticeConfig", "Engine", "Job", "Server", "library_path", "load_library", "device_info",
-           "copy_calibration", "autotune", "autotune_batch", "REFERENCE_TEMPERATURE"]
+           "copy_calibration", "autotune", "autotune_batch", "label_bonds_noalloc", "REFERENCE_TEMPERATURE"]
 
 FLIP_PARITY = 0   # flip a cluster iff its canonical root index is odd (the reference rule)
 FLIP_RANDOM = 1   # flip a cluster on one random bit of its root (MurmurHash3 of root, sweep, seed)
@@ -115,6 +115,7 @@ def load_library() -> ctypes.CDLL:
         "swmi_download_bonds": (ctypes.c_int, [vp, u32, vp]),
         "swmi_label_bonds": (ctypes.c_int, [vp, u32, vp]),
         "swmi_download_labels": (ctypes.c_int, [vp, u32, vp]),
+        "swmi_label_bonds_noalloc": (ctypes.c_int, [vp, u32, u32, vp, vp]),
         "swmi_rng_export": (ctypes.c_int, [vp, u32, vp, vp, vp, P(u32)]),
         "swmi_get_error_flags": (ctypes.c_int, [vp, u32, P(u32)]),
         "swmi_time_kernels": (ctypes.c_int, [vp, u32, P(f32)]),
@@ -159,6 +160,13 @@ def device_info() -> dict:
     return {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units,
             "wavefront_size": info.wavefront_size, "total_memory": info.total_memory, "l2_bytes": info.l2_bytes,
             "clock_khz": info.clock_khz}
+
+
+def label_bonds_noalloc(bonds_ptr: int, n0: int, n1: int, labels_ptr: int, stream: int = 0) -> None:
+    """Allocation-free labelling (swmi_label_bonds_noalloc): device bond bytes at bonds_ptr -> canonical uint32
+    labels at labels_ptr, enqueued on the HIP stream `stream` (0: default stream); the caller synchronises."""
+    _check(load_library().swmi_label_bonds_noalloc(ctypes.c_void_p(bonds_ptr), n0, n1, ctypes.c_void_p(labels_ptr),
+                                                   ctypes.c_void_p(stream)))
 
 
 def copy_calibration(nbytes: int, repeats: int = 10) -> float:

@@ -53,7 +53,8 @@ def _run(cmd: list[str]) -> None:
 
 def build_library() -> Path:
     LIB_DIR.mkdir(exist_ok=True)
-    sources = [str(CSRC / "swmi_kernels.hip"), str(CSRC / "swmi_engine.hip"), str(CSRC / "swmi_server.cpp")]
+    sources = [str(CSRC / "swmi_kernels.hip"), str(CSRC / "swmi_engine.hip"), str(CSRC / "swmi_noalloc.hip"),
+               str(CSRC / "swmi_server.cpp")]
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", str(ROOT / "include"), "-o", str(tmp), *sources])
     os.replace(tmp, LIB)

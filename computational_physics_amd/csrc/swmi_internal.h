@@ -151,6 +151,8 @@ namespace swmi
                            hipStream_t stream);
     // a lattice's spin array was replaced: no flip is pending
     void LaunchSetApplied(uint32_t* state, hipStream_t stream);
+    // allocation-free labelling: descriptor d into d_desc, a zero work list of n_blocks, fresh d.state
+    void LaunchNoAllocSetup(const LatDesc& d, LatDesc* d_desc, uint32_t* d_block_lat, uint32_t n_blocks, hipStream_t stream);
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t sample_mode,
                            hipStream_t stream);
     void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, uint32_t sample_mode, hipStream_t stream);

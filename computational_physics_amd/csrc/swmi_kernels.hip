@@ -1530,6 +1530,20 @@ namespace swmi
         state[kStLastSample] = it;
     }
 
+    // Set-up of the allocation-free labelling (swmi_label_bonds_noalloc): the descriptor, a work list of
+    // lattice 0 for every block and fresh counters, all written by this kernel into the caller's scratch.
+    __global__ __launch_bounds__(256) void k_noalloc_setup(const LatDesc d, LatDesc* __restrict__ desc_out,
+                                                           uint32_t* __restrict__ block_lat, const uint32_t n_blocks)
+    {
+        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i == 0)
+            *desc_out = d;
+        if (i < kLatStateWords)
+            d.state[i] = i == kStItEnd ? 0xFFFFFFFFu : (i == kStActive ? 1u : 0u);
+        for (uint32_t j = i; j < n_blocks; j += gridDim.x * blockDim.x)
+            block_lat[j] = 0u;
+    }
+
     // =============================================================================================
     // Initialisation
     // =============================================================================================
@@ -1733,6 +1747,12 @@ namespace swmi
     }
 
     void LaunchSetApplied(uint32_t* state, hipStream_t stream) { k_set_applied_one<<<1, 1, 0, stream>>>(state); }
+
+    void LaunchNoAllocSetup(const LatDesc& d, LatDesc* d_desc, uint32_t* d_block_lat, const uint32_t n_blocks, hipStream_t stream)
+    {
+        const uint32_t grid = std::min<uint32_t>((n_blocks + 255u) / 256u, 1024u);
+        k_noalloc_setup<<<std::max<uint32_t>(grid, 1u), 256, 0, stream>>>(d, d_desc, d_block_lat, n_blocks);
+    }
 
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t sample_mode,
                            hipStream_t stream)

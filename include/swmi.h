@@ -163,6 +163,20 @@ int swmi_download_bonds(swmi_engine* engine, uint32_t lattice, uint8_t* bonds);
 int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds);
 /* Canonical labels of the last labelling (after a sweep: the clusters that sweep flipped). */
 int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels);
+/* Allocation-free labelling. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Labels the bond field `bonds` (device memory, n0*n1 bytes, the format of swmi_label_bonds) of an
+ * n0 x n1 lattice (the extents swmi_lattice_config allows, at most 2^22 sites) and writes the canonical
+ * labels into `labels` (device memory, n0*n1 uint32), exactly as swmi_label_bonds + swmi_download_labels.
+ * The kernels are enqueued on `stream` (a hipStream_t; NULL is the default stream) and the call returns
+ * without waiting; it allocates nothing, issues no memset and copies nothing to or from the host. Every
+ * buffer beyond the two arrays comes from a 48 MiB static device array that each GPU holds until the
+ * process ends, shared by all calls on that GPU: calls that overlap on one GPU (other streams, other
+ * threads) race, calls on one stream are ordered. An unpadded lattice needs about 2.5 bytes of it per
+ * site; one whose rows are padded (swmi_get_layout) about 7 per padded site, and one that needs more than
+ * the array holds is rejected with the amount it needs. Errors in the arguments are reported before
+ * anything is enqueued. */
+int swmi_label_bonds_noalloc(const uint8_t* bonds, uint32_t n0, uint32_t n1, uint32_t* labels, void* stream);
 /* Per-site lcg32 states (n_groups*64, padded layout), selector bit planes (3*n_groups), rotations (n_groups)
  * and the iteration count of the lattice (n_groups: swmi_get_layout). */
 int swmi_rng_export(swmi_engine* engine, uint32_t lattice, uint32_t* states, uint64_t* masks, uint32_t* rot,

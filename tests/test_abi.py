@@ -110,3 +110,15 @@ def test_python_mirror_names():
         assert hasattr(cpa.Engine, name)
     assert cpa.REFERENCE_TEMPERATURE == pytest.approx(2.2691853, abs=1e-6)
 
+
+@pytest.mark.parametrize("n0,n1,why", [(63, 64, "even"), (2048, 4096, "at most"), (0, 64, "even")])
+def test_noalloc_labelling_rejects_bad_arguments_before_the_device(n0, n1, why):
+    """swmi_label_bonds_noalloc checks its arguments (extents, the 2^22-site limit of its static scratch
+    array) before it looks at a device; the arrays are never dereferenced on an error."""
+    import computational_physics_amd as cpa
+    lib = cpa.load_library()
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.swmi_label_bonds_noalloc(fake, n0, n1, fake, None) == 1  # SWMI_ERR_INVALID
+    assert why in lib.swmi_last_error().decode()
+    assert lib.swmi_label_bonds_noalloc(None, 64, 64, fake, None) == 1
+    assert lib.swmi_label_bonds_noalloc(fake, 64, 64, None, None) == 1
