@@ -374,6 +374,8 @@ void swmi_engine::Rebuild()
             it->twr_t = t;
         }
         d[i] = Desc(l, static_cast<uint32_t>(it->block_lat.size()));
+        if (it->n_lat++ == 0)
+            it->first_slot = i;
         it->block_lat.insert(it->block_lat.end(), l.g.n_blocks, i);
         it->max_twr = std::max(it->max_twr, l.g.twr);
     }
@@ -425,9 +427,9 @@ void swmi_engine::EnqueueSweeps(uint32_t n)
         hipStream_t s = ClassStream(c);
         for (uint32_t i = 0; i < n; ++i)
         {
-            LaunchBondLabel(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, k.pad, recording, s);
-            LaunchMerge(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
-            LaunchResolve(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, s);
+            LaunchBondLabel(k.Descs(d_desc.ptr), k.WorkList(), k.total_blocks, k.max_twr, k.tw64, k.pad, recording, s);
+            LaunchMerge(k.Descs(d_desc.ptr), k.WorkList(), k.total_blocks, s);
+            LaunchResolve(k.Descs(d_desc.ptr), k.WorkList(), k.total_blocks, s);
         }
     }
     for (size_t c = 1; c < nc; ++c)
@@ -441,7 +443,7 @@ void swmi_engine::EnqueueSweeps(uint32_t n)
 void swmi_engine::EnqueueMaterialize()
 {
     for (auto& k : classes)
-        LaunchMaterialize(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, stream);
+        LaunchMaterialize(k.Descs(d_desc.ptr), k.WorkList(), k.total_blocks, stream);
     LaunchCheck("flip launch");
 }
 
@@ -451,7 +453,7 @@ void swmi_engine::EnqueueSample(uint32_t mode)
         return;
     EnqueueMaterialize();
     for (auto& k : classes)
-        LaunchObservables(d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, mode, stream);
+        LaunchObservables(k.Descs(d_desc.ptr), k.WorkList(), k.total_blocks, mode, stream);
     LaunchFinalizeSamples(d_desc.ptr, static_cast<uint32_t>(descs.size()), mode, stream);
     LaunchCheck("sample launch");
 }
@@ -536,18 +538,13 @@ void swmi_engine::Run(uint32_t n, uint32_t sep, int32_t sample_mode)
     enqueue_unit(sampled ? total_groups : n);
 }
 
-// Work list holding only lattice l (block_base 0), for the inspection hooks.
+// Descriptor of lattice l alone (block_base 0) for the inspection hooks, which launch with a null work list.
 void swmi_engine::OneLattice(const Lattice& l, const uint8_t* bond_in)
 {
     LatDesc d = Desc(l, 0);
     d.bond_in = bond_in;
     if (!d_one_desc.ptr)
         d_one_desc.allocate(1);
-    if (d_zero_blocks.count < l.g.n_blocks)
-    {
-        d_zero_blocks.allocate(l.g.n_blocks);
-        Check(hipMemsetAsync(d_zero_blocks.ptr, 0, l.g.n_blocks * sizeof(uint32_t), stream), "work list");
-    }
     Check(hipMemcpyAsync(d_one_desc.ptr, &d, sizeof(d), hipMemcpyHostToDevice, stream), "descriptor upload");
     Check(hipStreamSynchronize(stream), "descriptor upload");
 }
@@ -555,7 +552,7 @@ void swmi_engine::OneLattice(const Lattice& l, const uint8_t* bond_in)
 void swmi_engine::MaterializeOne(const Lattice& l)
 {
     OneLattice(l, nullptr);
-    LaunchMaterialize(d_one_desc.ptr, d_zero_blocks.ptr, l.g.n_blocks, stream);
+    LaunchMaterialize(d_one_desc.ptr, nullptr, l.g.n_blocks, stream);
     LaunchCheck("flip launch");
 }
 
@@ -813,7 +810,7 @@ int swmi_observables(swmi_engine* engine, uint32_t lattice, int64_t* energy, int
     return Guard([&] {
         Lattice& l = Eng(engine)->At(lattice);
         engine->MaterializeOne(l);
-        LaunchObservables(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, kSampleAlways, engine->stream);
+        LaunchObservables(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, kSampleAlways, engine->stream);
         engine->LaunchCheck("observables launch");
         std::vector<int32_t> p(2ull * kWavesPerBlock * l.g.n_blocks);
         Check(hipMemcpyAsync(p.data(), l.partials.ptr, p.size() * 4, hipMemcpyDeviceToHost, engine->stream), "observables");
@@ -956,10 +953,10 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
         }
         engine->MaterializeOne(l);   // the injected pass replaces the labels a pending flip needs
         engine->OneLattice(l, l.bond_in.ptr);
-        LaunchBondLabel(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, l.np != l.n, true,
+        LaunchBondLabel(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, l.np != l.n, true,
                         engine->stream);
-        LaunchMerge(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, engine->stream);
-        LaunchWriteLabels(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, engine->stream);
+        LaunchMerge(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, engine->stream);
+        LaunchWriteLabels(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("label launch");
         Check(hipStreamSynchronize(engine->stream), "label");
     });
@@ -972,7 +969,7 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = Eng(engine)->At(lattice);
         engine->OneLattice(l, nullptr);
-        LaunchWriteLabels(engine->d_one_desc.ptr, engine->d_zero_blocks.ptr, l.g.n_blocks, l.g.twr, engine->stream);
+        LaunchWriteLabels(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("resolve launch");
         if (l.np == l.n)
             Check(hipMemcpyAsync(labels, l.parent.ptr, l.n * 4, hipMemcpyDeviceToHost, engine->stream), "label download");
@@ -1202,14 +1199,14 @@ int swmi_time_kernels(swmi_engine* engine, uint32_t n_sweeps, float* ms)
         {
             Check(hipEventRecord(ev[0], engine->stream), "event");
             for (auto& k : engine->classes)
-                LaunchBondLabel(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, k.max_twr, k.tw64, k.pad, engine->recording,
+                LaunchBondLabel(k.Descs(engine->d_desc.ptr), k.WorkList(), k.total_blocks, k.max_twr, k.tw64, k.pad, engine->recording,
                                 engine->stream);
             Check(hipEventRecord(ev[1], engine->stream), "event");
             for (auto& k : engine->classes)
-                LaunchMerge(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, engine->stream);
+                LaunchMerge(k.Descs(engine->d_desc.ptr), k.WorkList(), k.total_blocks, engine->stream);
             Check(hipEventRecord(ev[2], engine->stream), "event");
             for (auto& k : engine->classes)
-                LaunchResolve(engine->d_desc.ptr, k.d_block_lat.ptr, k.total_blocks, engine->stream);
+                LaunchResolve(k.Descs(engine->d_desc.ptr), k.WorkList(), k.total_blocks, engine->stream);
             Check(hipEventRecord(ev[3], engine->stream), "event");
             Check(hipEventSynchronize(ev[3]), "event sync");
             for (int k = 0; k < 3; ++k)

@@ -173,6 +173,12 @@ namespace swmi
         std::vector<uint32_t> block_lat;     // block -> lattice slot
         DeviceBuffer<uint32_t> d_block_lat;
         uint32_t total_blocks = 0;
+        uint32_t n_lat = 0;                  // lattices in the class
+        uint32_t first_slot = 0;             // descriptor slot of the class's first lattice
+        // The kernels' arguments for this class: a class of one lattice passes that lattice's descriptor and a
+        // null work list (the kernels then skip the work-list load), a larger class all descriptors and its list.
+        const LatDesc* Descs(const LatDesc* all) const { return n_lat == 1 ? all + first_slot : all; }
+        const uint32_t* WorkList() const { return n_lat == 1 ? nullptr : d_block_lat.ptr; }
     };
 
     struct GraphEntry
@@ -200,9 +206,8 @@ struct swmi_engine
     std::vector<hipStream_t> side_streams;
     std::vector<hipEvent_t> join_events;
     hipEvent_t fork_event = nullptr;
-    // single-lattice work lists for the inspection hooks
+    // single-lattice descriptor for the inspection hooks (launched with a null work list)
     swmi::DeviceBuffer<swmi::LatDesc> d_one_desc;
-    swmi::DeviceBuffer<uint32_t> d_zero_blocks;
 
     swmi_engine();
     ~swmi_engine();

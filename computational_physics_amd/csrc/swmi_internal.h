@@ -135,7 +135,8 @@ namespace swmi
 
     constexpr uint32_t kErrLabelRange = 1u;  // a label chase left [0, n) or met an unflagged root
 
-    // Launch wrappers (swmi_kernels.hip). Work list: block_lat[b] = lattice of global block b.
+    // Launch wrappers (swmi_kernels.hip). Work list: block_lat[b] = lattice of global block b; a null work list
+    // means every block belongs to descriptor 0.
     // all_tw64: every lattice of the work list has n0 % 64 == 0 (64-wide tiles); selects the specialised kernel
     // debug: compiles in injected bonds (LatDesc::bond_in) and bond recording (LatDesc::bond_dump)
     // pad: some lattice of the work list is padded (selects the PAD specialisation, correct for every lattice)
@@ -151,8 +152,8 @@ namespace swmi
                            hipStream_t stream);
     // a lattice's spin array was replaced: no flip is pending
     void LaunchSetApplied(uint32_t* state, hipStream_t stream);
-    // allocation-free labelling: descriptor d into d_desc, a zero work list of n_blocks, fresh d.state
-    void LaunchNoAllocSetup(const LatDesc& d, LatDesc* d_desc, uint32_t* d_block_lat, uint32_t n_blocks, hipStream_t stream);
+    // allocation-free labelling: descriptor d into d_desc, fresh d.state
+    void LaunchNoAllocSetup(const LatDesc& d, LatDesc* d_desc, hipStream_t stream);
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, uint32_t total_blocks, uint32_t sample_mode,
                            hipStream_t stream);
     void LaunchFinalizeSamples(const LatDesc* d_desc, uint32_t n_lattices, uint32_t sample_mode, hipStream_t stream);

@@ -218,6 +218,13 @@ namespace swmi
         return (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
     }
 
+    // Lattice of block b in the work list; a work list of one lattice is passed as a null pointer, which
+    // saves the dependent load of the block's entry before the descriptor.
+    __device__ __forceinline__ uint32_t LatticeOf(const uint32_t* __restrict__ block_lat, const uint32_t b)
+    {
+        return block_lat ? block_lat[b] : 0u;
+    }
+
     struct Block
     {
         uint32_t lb;       // block index within its lattice
@@ -352,7 +359,7 @@ namespace swmi
             ParamsOf(threadIdx.x, a, c);
             AC[threadIdx.x] = (static_cast<uint64_t>(c) << 32) | a;
         }
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
         const uint32_t lane4 = lane * 4u;
@@ -369,8 +376,53 @@ namespace swmi
         const bool rbound = lx == elane;
         const bool injected = DBG && d.bond_in != nullptr;
         const uint32_t thr = d.threshold;
-        // sweep schedule: a lattice past its last sweep is left alone by the whole sweep
         const gptr<uint32_t> stp = Glob(d.state);
+        const gptr<uint8_t> dump = DBG ? Glob(d.bond_dump) : nullptr;
+        const gptr<uint8_t> spins = Glob(d.spins);
+        const gptr<uint32_t> rotp = Glob(d.rot);
+        const gptr<const uint8_t> bond_in = Glob(d.bond_in);
+
+        const uint32_t r_begin = w * twr;
+        const uint32_t nrow = bl.nwr > r_begin ? min(twr, bl.nwr - r_begin) : 0u;
+        // buffer views of this block's part of the per-site arrays (row offsets in scalar registers, the lane
+        // offset in a vector register, no per-row vector addresses)
+        const uint32_t span = (bl.nwr - 1u) * bl.stride + 64u;   // sites from the block's first to its last
+        const Rsrc rs_spins = MakeRsrc(d.spins + bl.base, span);
+        const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
+        const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * twr;
+        const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
+        const bool last_wave = w + 1u == nwaves;
+        const uint32_t sx_r = bl.sx + 1u == d.tiles_x ? 0u : bl.sx + 1u;   // block column to the right
+        const uint32_t sy_b = bl.sy + 1u == d.blocks_y ? 0u : bl.sy + 1u;   // block row below
+        // the two sides of the halo array (by sweep parity)
+        const gptr<const uint8_t> halo_s0 = Glob(static_cast<const uint8_t*>(d.halo));
+        const gptr<const uint8_t> halo_s1 = halo_s0 + d.n_blocks * d.halo_stride;
+
+        // What needs only the descriptor and must be in place before the barrier is loaded while the lattice's
+        // counters are on their way, so that the two round trips overlap: the block's flip bits (they matter
+        // only under a pending flip) and every candidate for the spin below the tile (which one is used depends
+        // on the counters).
+        if (!injected)
+            for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
+                FB[i] = fbits_g[i];
+        const uint32_t y_last = (bl.wyb + r_begin + (nrow > 0 ? nrow : 1u) - 1u) * rpw + lrow;
+        const uint32_t below = (r_begin + nrow) * bl.stride + lx;   // the site below the tile, inside the block
+        const uint32_t below_halo = (sy_b * d.tiles_x + bl.sx) * d.halo_stride + d.ecol_stride + lx;
+        uint32_t hd_spin = 0, hd_lab = 0, hd_sv = 0, hd_h0 = 0, hd_h1 = 0;
+        if (nrow > 0)
+        {
+            hd_spin = spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
+            if (!injected)
+            {
+                hd_lab = __builtin_amdgcn_raw_buffer_load_b16(rs_local, below * 2u, 0, 0);
+                hd_sv = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, below, 0, 0);
+                hd_h0 = halo_s0[below_halo];
+                hd_h1 = halo_s1[below_halo];
+            }
+        }
+
+        // sweep schedule: a lattice past its last sweep is left alone by the whole sweep
         const uint32_t it0 = injected ? 0u : stp[kStIteration];
         const bool act = injected || it0 < stp[kStItEnd];
         if (bl.lb == 0 && threadIdx.x == 0)
@@ -391,28 +443,13 @@ namespace swmi
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
         const bool rot2 = !injected && (it2 % 15u == 0u);
-        const gptr<uint8_t> dump = DBG ? Glob(d.bond_dump) : nullptr;
-        const gptr<uint8_t> spins = Glob(d.spins);
-        const gptr<uint32_t> rotp = Glob(d.rot);
-        const gptr<const uint8_t> bond_in = Glob(d.bond_in);
 
-        const uint32_t r_begin = w * twr;
-        const uint32_t nrow = bl.nwr > r_begin ? min(twr, bl.nwr - r_begin) : 0u;
-        // buffer views of this block's part of the per-site arrays (row offsets in scalar registers, the lane
-        // offset in a vector register, no per-row vector addresses)
-        const uint32_t span = (bl.nwr - 1u) * bl.stride + 64u;   // sites from the block's first to its last
-        const Rsrc rs_spins = MakeRsrc(d.spins + bl.base, span);
-        const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
-        const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
-        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * twr;
         // halos: this sweep writes the block's first column and first wave row into its side of the halo
         // array; the other side holds the neighbours' flipped halos of the pending flip
         const uint32_t hside = (it0 >> 1) & 1u;
-        const gptr<const uint8_t> halo_g = Glob(static_cast<const uint8_t*>(d.halo)) + (hside ^ 1u) * d.n_blocks * d.halo_stride;
+        const gptr<const uint8_t> halo_g = hside ? halo_s0 : halo_s1;
         const Rsrc rs_halo = MakeRsrc(d.halo + (hside * d.n_blocks + bl.lb) * d.halo_stride, d.halo_stride);
         const uint32_t hcol_off = (lx == 0u && !injected) ? lrow : kNoStore;   // first column: one lane per lattice row
-        const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
-        const bool last_wave = w + 1u == nwaves;
         uint32_t* const Qw = Q + w * kQw;
         uint32_t* const Pl = At(P, lane4);   // this lane's entry of wave row 0
         uint32_t* const dummy = Qw + 64u + lane;
@@ -438,11 +475,8 @@ namespace swmi
         const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
         const uint64_t lanes_below = (1ull << lane) - 1ull;
 
-        // pending flip: the block's flip bits, one per block-local root index (k_resolve wrote the bit of
-        // every root, edge-reaching or not), so a site's flip is the bit at its label's root index
-        if (pending)
-            for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
-                FB[i] = fbits_g[i];
+        // pending flip: the block's flip bits (loaded into FB above), one per block-local root index (k_resolve
+        // wrote the bit of every root, edge-reaching or not), so a site's flip is the bit at its label's root
         const auto flip_of = [&](const uint32_t lab) { return (LdsLoad(&FB[(lab & 0x7FFFu) >> 5]) >> (lab & 31u)) & 1u; };
 
         // spins below the tile (down halo), needed last but fetched first. Under a pending flip, below the
@@ -454,21 +488,12 @@ namespace swmi
         const bool carry = nrow > 0 && pending && !last_wave;
         if (nrow > 0)
         {
-            const uint32_t y_last = (bl.wyb + r_begin + nrow - 1u) * rpw + lrow;
             if (!pending)
-                halo_down = spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
+                halo_down = hd_spin;
             else if (carry)
-            {
-                const uint32_t below = (r_begin + nrow) * bl.stride + lx;
-                const uint32_t lab = __builtin_amdgcn_raw_buffer_load_b16(rs_local, below * 2u, 0, 0);
-                const uint32_t sv = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, below, 0, 0);
-                __hip_atomic_store(dummy, (lab << 8) | (sv & 0xFFu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+                __hip_atomic_store(dummy, (hd_lab << 8) | (hd_sv & 0xFFu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             else
-            {
-                const uint32_t sy_b = bl.sy + 1u == d.blocks_y ? 0u : bl.sy + 1u;
-                halo_down = halo_g[(sy_b * d.tiles_x + bl.sx) * d.halo_stride + d.ecol_stride + lx];
-            }
+                halo_down = hside ? hd_h0 : hd_h1;
         }
         __syncthreads();   // AC, FB, the carried site below
         if (carry)
@@ -494,38 +519,40 @@ namespace swmi
             constexpr bool kPend = decltype(flipping)::value;
             // rows loaded together: a flip on load also keeps the labels in flight, so half as many
             constexpr uint32_t kC = kPend && kCh > 1u ? kCh / 2u : kCh;
-            // a tile of at most kC rows is one chunk, starting at row 0
-            for (uint32_t r0 = 0; r0 < (TWR_MAX <= kC ? 1u : nrow); r0 += kC)
+            // what one chunk of rows loads per lane: the spin byte, the lcg32 state and, under a pending flip,
+            // the label of its site in each row; one right-halo spin and one rotation offset per row (lane i:
+            // row r0 + i of the chunk)
+            struct Chunk
             {
-                // The loads carry no condition (rows past the tile re-read its last row, values unused), so
-                // that they issue back to back and their latencies overlap.
-                // lane i: row r0 + i of the chunk (for the per-row values fetched one per lane)
+                uint32_t sp[kC], aux[kC], lab[kC], h, g_rot;
+            };
+            // The loads carry no condition (rows past the tile re-read its last row, values unused), so that
+            // they issue back to back and their latencies overlap.
+            const auto load = [&](const uint32_t r0, Chunk& c) {
                 const uint32_t lrow_ch = r_begin + min(r0 + (lane % kC), nrow - 1u);
-                // right halo of a 64-wide tile: lane i fetches the column-64 spin of row r0 + i
-                uint32_t h = 0;
+                // right halo of a 64-wide tile: under a pending flip the right neighbour's flipped first column
+                c.h = 0;
                 if (TW64)
-                {
-                    const uint32_t sx_r = bl.sx + 1u == d.tiles_x ? 0u : bl.sx + 1u;
-                    if (pending)   // the right neighbour's flipped first column
-                        h = halo_g[(bl.sy * d.tiles_x + sx_r) * d.halo_stride + lrow_ch];
-                    else
-                        h = spins[(bl.wyb + lrow_ch) * n0 + sx_r * 64u];
-                }
+                    c.h = pending ? halo_g[(bl.sy * d.tiles_x + sx_r) * d.halo_stride + lrow_ch]
+                                  : spins[(bl.wyb + lrow_ch) * n0 + sx_r * 64u];
                 // rotation offsets of the chunk's groups, needed on rotating sweeps only
-                uint32_t g_rot = 0;
-                if (kRot)
-                    g_rot = rotp[(bl.base + lrow_ch * bl.stride) >> 6];
-                uint32_t sp[kC], aux[kC], lab[kC];
+                c.g_rot = kRot ? rotp[(bl.base + lrow_ch * bl.stride) >> 6] : 0u;
 #pragma unroll
                 for (uint32_t i = 0; i < kC; ++i)
                 {
                     // row base in scalar registers, the lane offset in a vector register
                     const uint32_t wr = r_begin + min(r0 + i, nrow - 1u);
                     const uint32_t rowoff = Uniform(wr * bl.stride);
-                    sp[i] = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, lane, rowoff, 0);
-                    aux[i] = injected ? (bond_in + bl.base + rowoff)[lane] : __builtin_amdgcn_raw_buffer_load_b32(rs_rng, lane4, rowoff * 4u, 0);
-                    lab[i] = kPend ? __builtin_amdgcn_raw_buffer_load_b16(rs_local, lane * 2u, rowoff * 2u, 0) : 0u;
+                    c.sp[i] = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, lane, rowoff, 0);
+                    c.aux[i] = injected ? (bond_in + bl.base + rowoff)[lane] : __builtin_amdgcn_raw_buffer_load_b32(rs_rng, lane4, rowoff * 4u, 0);
+                    c.lab[i] = kPend ? __builtin_amdgcn_raw_buffer_load_b16(rs_local, lane * 2u, rowoff * 2u, 0) : 0u;
                 }
+            };
+            const auto process = [&](const uint32_t r0, Chunk& c) {
+                uint32_t(&sp)[kC] = c.sp;
+                const uint32_t(&aux)[kC] = c.aux;
+                const uint32_t(&lab)[kC] = c.lab;
+                const uint32_t h = c.h, g_rot = c.g_rot;
                 if (kPend)
                 {
 #pragma unroll
@@ -693,6 +720,13 @@ namespace swmi
                         prev_site = site;
                     }
                 }
+            };
+            // a tile of at most kC rows is one chunk, starting at row 0
+            for (uint32_t r0 = 0; r0 < (TWR_MAX <= kC ? 1u : nrow); r0 += kC)
+            {
+                Chunk c;
+                load(r0, c);
+                process(r0, c);
             }
             };
             if (rot1 || rot2)
@@ -987,7 +1021,7 @@ namespace swmi
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         if (Glob(d.state)[kStActive] == 0u)
             return;
         const Block bl = DecodeBlock(d, b);
@@ -1136,7 +1170,7 @@ namespace swmi
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         const gptr<uint32_t> stp = Glob(d.state);
         if (stp[kStActive] == 0u)
             return;
@@ -1216,7 +1250,7 @@ namespace swmi
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         const gptr<uint32_t> stp = Glob(d.state);
         if (stp[kStApplied] == stp[kStIteration])
             return;
@@ -1284,7 +1318,7 @@ namespace swmi
         const uint32_t b = blockIdx.x * 256u + threadIdx.x;
         if (b >= total_blocks)
             return;
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         if (b == d.block_base)
             d.state[kStApplied] = d.state[kStIteration];
     }
@@ -1305,7 +1339,7 @@ namespace swmi
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         const Block bl = DecodeBlock(d, b);
         const gptr<uint32_t> L = Glob(d.parent);
         const gptr<uint16_t> local = Glob(d.local);
@@ -1415,7 +1449,7 @@ namespace swmi
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
-        const LatDesc& d = descs[block_lat[b]];
+        const LatDesc& d = descs[LatticeOf(block_lat, b)];
         if (sample_mode == kSampleDue && !SampleDue(d.state))
             return;
         const Block bl = DecodeBlock(d, b);
@@ -1530,18 +1564,15 @@ namespace swmi
         state[kStLastSample] = it;
     }
 
-    // Set-up of the allocation-free labelling (swmi_label_bonds_noalloc): the descriptor, a work list of
-    // lattice 0 for every block and fresh counters, all written by this kernel into the caller's scratch.
-    __global__ __launch_bounds__(256) void k_noalloc_setup(const LatDesc d, LatDesc* __restrict__ desc_out,
-                                                           uint32_t* __restrict__ block_lat, const uint32_t n_blocks)
+    // Set-up of the allocation-free labelling (swmi_label_bonds_noalloc): the descriptor and fresh counters,
+    // written by this kernel into the scratch memory.
+    __global__ __launch_bounds__(64) void k_noalloc_setup(const LatDesc d, LatDesc* __restrict__ desc_out)
     {
-        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t i = threadIdx.x;
         if (i == 0)
             *desc_out = d;
         if (i < kLatStateWords)
             d.state[i] = i == kStItEnd ? 0xFFFFFFFFu : (i == kStActive ? 1u : 0u);
-        for (uint32_t j = i; j < n_blocks; j += gridDim.x * blockDim.x)
-            block_lat[j] = 0u;
     }
 
     // =============================================================================================
@@ -1748,10 +1779,10 @@ namespace swmi
 
     void LaunchSetApplied(uint32_t* state, hipStream_t stream) { k_set_applied_one<<<1, 1, 0, stream>>>(state); }
 
-    void LaunchNoAllocSetup(const LatDesc& d, LatDesc* d_desc, uint32_t* d_block_lat, const uint32_t n_blocks, hipStream_t stream)
+    void LaunchNoAllocSetup(const LatDesc& d, LatDesc* d_desc, hipStream_t stream)
     {
-        const uint32_t grid = std::min<uint32_t>((n_blocks + 255u) / 256u, 1024u);
-        k_noalloc_setup<<<std::max<uint32_t>(grid, 1u), 256, 0, stream>>>(d, d_desc, d_block_lat, n_blocks);
+        static_assert(kLatStateWords <= 64, "one wave writes the counters");
+        k_noalloc_setup<<<1, 64, 0, stream>>>(d, d_desc);
     }
 
     void LaunchObservables(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, const uint32_t sample_mode,

@@ -3,8 +3,8 @@
 // hipMalloc).
 //
 // The caller passes the bond field and the label array in device memory. Every other buffer the kernels
-// need (block-local labels, block-edge bond bits, edge-root lists, edge columns, per-lattice counters, the
-// lattice descriptor and the block work list) is carved out of one file-scope device array, `scratch`,
+// need (block-local labels, block-edge bond bits, edge-root lists, edge columns, per-lattice counters and
+// the lattice descriptor) is carved out of one file-scope device array, `scratch`,
 // which every GPU holds for the life of the process. For a lattice that needs no row padding the global
 // union-find runs in the caller's label array itself, which the last kernel overwrites with the canonical
 // labels; a padded lattice gets its bond field padded into scratch and its union-find in scratch, and a
@@ -78,13 +78,12 @@ namespace swmi
         // The label-only descriptor over a carve (spins alias the bond field: the injected pass reads spin
         // bytes but uses none of them; it reads no lcg32 state and writes no spin, halo or flip bit).
         LatDesc NoAllocDesc(const Geometry& g, const uint32_t n0, const uint32_t n1, const uint8_t* bonds, uint32_t* labels,
-                            Carve& c, LatDesc** d_desc, uint32_t** d_block_lat)
+                            Carve& c, LatDesc** d_desc)
         {
             LatDesc d = GeometryDesc(g, n0, n1);
             const uint64_t n = static_cast<uint64_t>(g.pitch) * n1;   // padded sites
             const bool pad = g.pitch != n0;
             *d_desc = c.Take<LatDesc>(1);
-            *d_block_lat = c.Take<uint32_t>(g.n_blocks);
             d.state = c.Take<uint32_t>(kLatStateWords);
             d.local = c.Take<uint16_t>(n);
             d.bbits = c.Take<uint64_t>(static_cast<size_t>(kBbitWords) * g.n_blocks);
@@ -117,8 +116,7 @@ extern "C" int swmi_label_bonds_noalloc(const uint8_t* bonds, uint32_t n0, uint3
         const Geometry g = GeometryOf(n0, n1, kNoAllocTileRows);
         Carve size;
         LatDesc* d_desc = nullptr;
-        uint32_t* d_block_lat = nullptr;
-        (void)NoAllocDesc(g, n0, n1, bonds, labels, size, &d_desc, &d_block_lat);
+        (void)NoAllocDesc(g, n0, n1, bonds, labels, size, &d_desc);
         if (size.used > kScratchBytes)
             throw SwmiError(SWMI_ERR_INVALID, "lattice needs " + std::to_string(size.used) + " bytes of scratch, more than the " +
                                                   std::to_string(kScratchBytes) + " the static array holds");
@@ -128,17 +126,18 @@ extern "C" int swmi_label_bonds_noalloc(const uint8_t* bonds, uint32_t n0, uint3
             throw SwmiError(SWMI_ERR_NO_DEVICE, "device index " + std::to_string(device) + " out of range");
         Carve carve;
         carve.base = ScratchOf(device);
-        const LatDesc d = NoAllocDesc(g, n0, n1, bonds, labels, carve, &d_desc, &d_block_lat);
+        const LatDesc d = NoAllocDesc(g, n0, n1, bonds, labels, carve, &d_desc);
         hipStream_t s = static_cast<hipStream_t>(stream);
         const bool pad = g.pitch != n0;
         const uint64_t np = static_cast<uint64_t>(g.pitch) * n1, n = static_cast<uint64_t>(n0) * n1;
-        // descriptor, zero work list and fresh counters are written by a kernel (no memset, no host copy)
-        LaunchNoAllocSetup(d, d_desc, d_block_lat, g.n_blocks, s);
+        // descriptor and fresh counters are written by a kernel (no memset, no host copy); the kernels run with
+        // a null work list (every block belongs to the one lattice)
+        LaunchNoAllocSetup(d, d_desc, s);
         if (pad)
             LaunchPadBytes(const_cast<uint8_t*>(d.bond_in), bonds, np, n0, g.pitch, s);
-        LaunchBondLabel(d_desc, d_block_lat, g.n_blocks, g.twr, g.tw == 64, pad, true, s);
-        LaunchMerge(d_desc, d_block_lat, g.n_blocks, s);
-        LaunchWriteLabels(d_desc, d_block_lat, g.n_blocks, g.twr, s);
+        LaunchBondLabel(d_desc, nullptr, g.n_blocks, g.twr, g.tw == 64, pad, true, s);
+        LaunchMerge(d_desc, nullptr, g.n_blocks, s);
+        LaunchWriteLabels(d_desc, nullptr, g.n_blocks, g.twr, s);
         if (pad)
             LaunchUnpadLabels(labels, d.parent, n, n0, g.pitch, s);
         Check(hipGetLastError(), "label launch");

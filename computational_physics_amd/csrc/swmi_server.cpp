@@ -205,7 +205,7 @@ void swmi_server::Retire(Lane& lane, ActiveJob& a)
     Check(hipMemcpyAsync(&st, a.lat->state.ptr, sizeof(st), hipMemcpyDeviceToHost, e.stream), "job result");
     // final configuration's (E, M), reduced over this lattice only
     e.MaterializeOne(*a.lat);
-    LaunchObservables(e.d_one_desc.ptr, e.d_zero_blocks.ptr, a.lat->g.n_blocks, kSampleAlways, e.stream);
+    LaunchObservables(e.d_one_desc.ptr, nullptr, a.lat->g.n_blocks, kSampleAlways, e.stream);
     Check(hipGetLastError(), "job observables");
     std::vector<int32_t> part(2ull * kWavesPerBlock * a.lat->g.n_blocks);
     Check(hipMemcpyAsync(part.data(), a.lat->partials.ptr, part.size() * 4, hipMemcpyDeviceToHost, e.stream), "job result");
