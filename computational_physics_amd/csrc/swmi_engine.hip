@@ -161,6 +161,8 @@ namespace swmi
         g.edge_cap = std::min<uint32_t>(g.twr * 64u, 2u * g.tw + 2u * kWavesPerBlock * g.twr * g.rpw);
         g.ecol_stride = kWavesPerBlock * g.twr * g.rpw;
         g.halo_stride = g.ecol_stride + 64u;
+        g.qcap = kWavesPerBlock * g.twr * 64u;
+        g.fbw = (g.qcap + kWavesPerBlock * g.edge_cap + 31u) / 32u;
         return g;
     }
 
@@ -182,6 +184,8 @@ namespace swmi
         d.nwr_y = g.nwr_y;
         d.n_blocks = g.n_blocks;
         d.halo_stride = g.halo_stride;
+        d.qcap = g.qcap;
+        d.fbw = g.fbw;
         return d;
     }
 
@@ -189,10 +193,10 @@ namespace swmi
     {
         const Geometry g = GeometryOf(c.n0, c.n1, c.tile_rows);
         const uint64_t n = static_cast<uint64_t>(g.pitch) * c.n1, groups = n / 64;
-        return n * (1 + 4 + 2 + 4) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
+        return n * (1 + 4 + 2) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
                4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
-               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (2ull * g.edge_cap + 4) + 4ull * g.ecol_stride * g.n_blocks +
-               32ull * g.twr * g.n_blocks + 2ull * g.halo_stride * g.n_blocks;
+               static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (8ull * g.edge_cap + 4) + 8ull * g.ecol_stride * g.n_blocks +
+               4ull * 2 * 64 * g.n_blocks + 4ull * g.fbw * g.n_blocks + 2ull * g.halo_stride * g.n_blocks;
     }
 
     bool Lattice::SameShape(const swmi_lattice_config& c) const
@@ -203,8 +207,8 @@ namespace swmi
 
     uint64_t Lattice::Bytes() const
     {
-        return spins.bytes() + rng.bytes() + masks.bytes() + rot.bytes() + local.bytes() + parent.bytes() + bbits.bytes() +
-               state.bytes() + partials.bytes() + edge_list.bytes() + edge_count.bytes() + ecol.bytes() + fbits.bytes() +
+        return spins.bytes() + rng.bytes() + masks.bytes() + rot.bytes() + local.bytes() + parent.bytes() + cpar.bytes() +
+               bbits.bytes() + state.bytes() + partials.bytes() + edge_count.bytes() + ecol.bytes() + erow.bytes() + fbits.bytes() +
                halo.bytes() + bond_dump.bytes() + bond_in.bytes();
     }
 
@@ -220,15 +224,16 @@ namespace swmi
         masks.allocate(3ull * n_groups);
         rot.allocate(n_groups);
         local.allocate(np);
-        parent.allocate(np);
+        parent.reset();
+        cpar.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
         bbits.allocate(static_cast<size_t>(kBbitWords) * g.n_blocks);
         state_words = kLatStateWords + 4u * c.max_samples;
         state.allocate(state_words);
         partials.allocate(2ull * kWavesPerBlock * g.n_blocks);
-        edge_list.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
         edge_count.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
         ecol.allocate(2ull * g.ecol_stride * g.n_blocks);
-        fbits.allocate(8ull * g.twr * g.n_blocks);
+        erow.allocate(2ull * 64 * g.n_blocks);
+        fbits.allocate(static_cast<size_t>(g.fbw) * g.n_blocks);
         halo.allocate(2ull * g.halo_stride * g.n_blocks);
     }
 
@@ -247,7 +252,6 @@ namespace swmi
             Check(hipMemsetAsync(spins.ptr, 0, np, stream), "spin init");
         LaunchHotStart(spins.ptr, n, c.spin_seed, cfg.n0, g.pitch, stream);
         LaunchInitRng(spins.ptr, rng.ptr, masks.ptr, rot.ptr, n_groups, c.seed, stream);
-        LaunchIota(parent.ptr, np, stream);
         Check(hipMemsetAsync(local.ptr, 0, local.bytes(), stream), "label init");
         Check(hipGetLastError(), "initialisation launch");
     }
@@ -322,14 +326,15 @@ LatDesc swmi_engine::Desc(const Lattice& l, uint32_t block_base) const
     d.rot = l.rot.ptr;
     d.local = l.local.ptr;
     d.parent = l.parent.ptr;
+    d.cpar = l.cpar.ptr;
     d.bbits = l.bbits.ptr;
     d.state = l.state.ptr;
     d.bond_dump = l.bond_dump.ptr;
     d.bond_in = nullptr;
     d.partials = l.partials.ptr;
-    d.edge_list = l.edge_list.ptr;
     d.edge_count = l.edge_count.ptr;
     d.ecol = l.ecol.ptr;
+    d.erow = l.erow.ptr;
     d.threshold = static_cast<uint32_t>(l.threshold);
     d.block_base = block_base;
     d.flip_rule = l.cfg.flip_rule;
@@ -952,6 +957,8 @@ int swmi_label_bonds(swmi_engine* engine, uint32_t lattice, const uint8_t* bonds
             Check(hipStreamSynchronize(engine->stream), "bond upload");
         }
         engine->MaterializeOne(l);   // the injected pass replaces the labels a pending flip needs
+        if (!l.parent.ptr)
+            l.parent.allocate(l.np);
         engine->OneLattice(l, l.bond_in.ptr);
         LaunchBondLabel(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, l.g.twr, l.g.tw == 64, l.np != l.n, true,
                         engine->stream);
@@ -968,6 +975,8 @@ int swmi_download_labels(swmi_engine* engine, uint32_t lattice, uint32_t* labels
         if (!labels)
             throw SwmiError(SWMI_ERR_INVALID, "null argument");
         Lattice& l = Eng(engine)->At(lattice);
+        if (!l.parent.ptr)
+            l.parent.allocate(l.np);
         engine->OneLattice(l, nullptr);
         LaunchWriteLabels(engine->d_one_desc.ptr, nullptr, l.g.n_blocks, l.g.twr, engine->stream);
         engine->LaunchCheck("resolve launch");

@@ -110,7 +110,7 @@ namespace swmi
     struct Geometry
     {
         uint32_t pitch = 0, tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0,
-                 ecol_stride = 0, halo_stride = 0, elast = 0;
+                 ecol_stride = 0, halo_stride = 0, elast = 0, qcap = 0, fbw = 0;
         uint64_t vlast = 0;
     };
     // Row pitch of an n0 x n1 lattice: n0 when the wave64 tiling takes it as it is, else the next width it
@@ -142,13 +142,14 @@ namespace swmi
         DeviceBuffer<uint64_t> masks;
         DeviceBuffer<uint32_t> rot;
         DeviceBuffer<uint16_t> local;
-        DeviceBuffer<uint32_t> parent;
+        DeviceBuffer<uint32_t> parent;       // inspection labels, allocated by the first call that needs them
+        DeviceBuffer<uint64_t> cpar;
         DeviceBuffer<uint64_t> bbits;
         DeviceBuffer<uint32_t> state;
         DeviceBuffer<int32_t> partials;
-        DeviceBuffer<uint16_t> edge_list;
         DeviceBuffer<uint32_t> edge_count;
-        DeviceBuffer<uint16_t> ecol;
+        DeviceBuffer<uint32_t> ecol;
+        DeviceBuffer<uint32_t> erow;
         DeviceBuffer<uint32_t> fbits;
         DeviceBuffer<uint8_t> halo;
         DeviceBuffer<uint8_t> bond_dump;

@@ -12,17 +12,21 @@
 //   rng      uint32[n]       per-site lcg32 state (lane i of group g = site 64g+i)
 //   masks    uint64[3*n/64]  initial (a,c) selector of each lane as three bit planes (read-only)
 //   rot      uint32[n/64]    rotation offset of each group's (a,c) arrays (updated on rotating sweeps)
-//   local    uint16[n]       block-local root of each site | 0x8000 if its cluster reaches the block edge
-//   parent   uint32[n]       global union-find over the roots of edge-reaching clusters (other entries
-//                            unused); after a resolve pass, site -> canonical (minimum) site index
+//   local    uint16[n]       block label of each site: the block-local index q of its cluster's root, or for a
+//                            cluster that reaches the block edge 0x8000 | (qcap + s), s = the root's slot
+//   cpar     uint64[4*edge_cap*blocks]  global union-find over the roots of edge-reaching clusters, one
+//                            entry per slot: slot s of tile t of block b is c = (4b + t) * edge_cap + s', its
+//                            entry (key << 32) | parent slot, key = the root's site index (the order of the
+//                            union-find: a parent's key never exceeds its child's, so a tree's root is the
+//                            minimum site of the component). Each tile numbers its edge roots densely
+//   edge_count uint32[4*blocks]  slots in use in each tile
 //   bbits    uint64[5*blocks] block-edge bond bits: down edge, then right edge (64 rows per word)
-//   edge_list  uint16[4*blocks*edge_cap]  per tile: block-local indices of the roots of its clusters
-//                            that reach the block edge (the only roots the resolve pass must chase)
-//   edge_count uint32[4*blocks]           entries of each tile's list
-//   ecol     uint16[2*ecol_stride*blocks] labels of each block's first (side 0) and last (side 1) column,
-//                            one entry per lattice row of the block, for the cross-block merge
-//   fbits    uint32[8*twr*blocks]   per block: flip bit of each edge-reaching root, by block-local index
-//                            (written by k_resolve, read by the next sweep's k_bond_label)
+//   ecol     uint32[2*ecol_stride*blocks] (label << 16) | root index q of each block's first (side 0) and
+//                            last (side 1) column, one entry per lattice row of the block, for the merge
+//   erow     uint32[2*64*blocks] the same for the block's first (side 0) and last (side 1) wave row
+//   fbits    uint32[fbw*blocks] per block: the flip bit of every cluster by its label's index (root index q,
+//                            or qcap + slot), written by k_resolve, read by the next sweep's k_bond_label
+//   parent   uint32[n]       inspection only: canonical (minimum) site index of every site (k_write_labels)
 //   halo     uint8[2*(ecol_stride+64)*blocks] two sides (by sweep parity) of, per block, the spins of its
 //                            first column (one per lattice row) and of its first wave row: k_bond_label
 //                            writes them into the sweep's side, k_resolve flips them in place, and in the
@@ -66,17 +70,20 @@ namespace swmi
         const uint64_t* masks;
         uint32_t* rot;
         uint16_t* local;
-        uint32_t* parent;
+        uint32_t* parent;        // inspection output of k_write_labels
+        uint64_t* cpar;          // slot union-find
         uint64_t* bbits;         // kBbitWords words per block
         uint32_t* state;         // per-lattice device counters, see LatState
         uint8_t* bond_dump;      // per-site bond bytes of the last sweep (nullptr: not recorded)
         const uint8_t* bond_in;  // injected bond bytes (label-only mode), nullptr otherwise
         int32_t* partials;       // 2 int32 (E, M) per tile for observables
-        uint16_t* edge_list;     // kWavesPerBlock lists of edge_cap entries per block
         uint32_t* edge_count;    // kWavesPerBlock counts per block
-        uint16_t* ecol;          // block edge-column labels: [block][side][lattice row in block]
+        uint32_t* ecol;          // block edge columns: [block][side][lattice row in block] of (label << 16) | q
+        uint32_t* erow;          // block edge wave rows: [block][side][lane] of (label << 16) | q
         uint32_t ecol_stride;    // lattice rows per block: kWavesPerBlock * twr * rpw
-        uint32_t edge_cap;       // capacity of one tile's list: min(tile sites, block boundary sites)
+        uint32_t edge_cap;       // slots of one tile: min(tile sites, block boundary sites)
+        uint32_t qcap;           // block-local indices of a block: kWavesPerBlock * twr * 64
+        uint32_t fbw;            // flip-bit words per block: (qcap + 4 * edge_cap) / 32, rounded up
         uint32_t n0, n1;         // row pitch (padded width) and height
         uint32_t n0v;            // lattice width: columns [n0v, n0) are padding
         uint32_t elast;          // lane-in-tile (lx) of a row's last site in the last tile column
@@ -92,7 +99,7 @@ namespace swmi
         uint32_t n_blocks;       // tiles_x * blocks_y
         uint32_t flip_rule;      // SWMI_FLIP_PARITY (0) or SWMI_FLIP_RANDOM (1)
         uint32_t seed;           // the lattice's lcg32 seed (keys the random flip rule)
-        uint32_t* fbits;         // 8 * twr words per block
+        uint32_t* fbits;         // fbw words per block
         uint8_t* halo;           // two sides of n_blocks * halo_stride bytes: first column, then first wave row
         uint32_t halo_stride;    // ecol_stride + 64
     };

@@ -129,47 +129,69 @@ namespace swmi
         (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
-    // ---- union-find in HBM over the roots of edge-reaching clusters. Links are made with agent-scope
-    // atomicMin (the 8 XCD L2s are not coherent with each other for plain stores). Chases read through
-    // the caches (workgroup-scope relaxed loads: never hoisted by the compiler, allowed to hit L1/L2); a
-    // stale value is an earlier parent, i.e. still an ancestor, and every link is decided by an atomicMin
-    // that returns the true value, so staleness only costs retries. ----
-    __device__ __forceinline__ uint32_t CachedLoad(gptr<uint32_t> p)
+    // ---- union-find in HBM over the roots of edge-reaching clusters, one node per slot (every block numbers
+    // its edge roots densely, so a block's nodes share a few cache lines). Entry of slot c: (key << 32) | parent
+    // slot, where a root's key is its site index and a non-root holds its parent's key. Links are made with
+    // agent-scope 64-bit atomicMin (the 8 XCD L2s are not coherent with each other for plain stores), which
+    // orders by key first: a parent's key never exceeds its child's, so a tree's root is the minimum site of
+    // its component. Chases read through the caches (workgroup-scope relaxed loads: never hoisted by the
+    // compiler, allowed to hit L1/L2); a stale value is an earlier parent, still an ancestor, and every link is
+    // decided by an atomicMin that returns the true value, so staleness only costs retries. ----
+    __device__ __forceinline__ uint64_t SlotEntry(const uint32_t key, const uint32_t slot)
+    {
+        return (static_cast<uint64_t>(key) << 32) | slot;
+    }
+
+    __device__ __forceinline__ uint64_t CachedLoad64(gptr<uint64_t> p)
     {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
-    // Parents satisfy L[x] <= x, so a chase ends within n steps; a value outside [0, n) can only come
-    // from a broken invariant, which is reported through the lattice's error flag instead of being
-    // dereferenced. Both ends are chased together (their loads in flight at once); a start node more
-    // than one hop from its root is pointed at the root (the atomic returns nothing, so the wave does not
-    // wait for it). Then the larger root is linked under the smaller, and a lost race continues with the
-    // value found (the loop of the reference Merge, swendsen_wang_amd_gpu.cpp:368-401).
-    // The ends start as the edge roots k_bond_label listed (parent = self), so the first link is tried at
-    // once, without chasing: when a concurrent union got there first, atomicMin returns the parent found
-    // and the loop continues from it.
-    __device__ void GlobalUnion(gptr<uint32_t> L, uint32_t a, uint32_t b, const uint32_t n, gptr<uint32_t> err)
+    __device__ __forceinline__ uint64_t SlotLinkMin(gptr<uint64_t> p, const uint64_t v)
     {
-        if (a < n && b < n && a != b)
+        return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // Union of the clusters of slots a and b (keys ka, kb), the loop of the reference Merge
+    // (swendsen_wang_amd_gpu.cpp:368-401) on slot entries. The ends start as the listed roots k_bond_label
+    // wrote (entry = (own key << 32) | self), so the first link is tried at once, without chasing: when a
+    // concurrent union got there first, atomicMin returns the parent found and the loop continues from it.
+    // Both ends are chased together (their loads in flight at once); a start node more than one hop from its
+    // root is pointed at the root (the atomic returns nothing, so the wave does not wait for it). A slot out
+    // of range or a parent with a larger key can only come from a broken invariant, which is reported through
+    // the lattice's error flag instead of being followed.
+    __device__ void SlotUnion(gptr<uint64_t> E, uint32_t a, uint32_t ka, uint32_t b, uint32_t kb, const uint32_t nslots,
+                              gptr<uint32_t> err)
+    {
+        if (a < nslots && b < nslots && ka != kb)
         {
-            const uint32_t lo = min(a, b), hi = max(a, b);
-            const uint32_t old = __hip_atomic_fetch_min(&L[hi], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old == hi || old == lo)
+            if (ka > kb)
+            {
+                const uint32_t t = a, tk = ka;
+                a = b;
+                ka = kb;
+                b = t;
+                kb = tk;
+            }
+            const uint64_t old = SlotLinkMin(&E[b], SlotEntry(ka, a));
+            if (old == SlotEntry(kb, b) || static_cast<uint32_t>(old) == a)
                 return;
-            a = lo;
-            b = old;
+            b = static_cast<uint32_t>(old);
+            kb = static_cast<uint32_t>(old >> 32);
         }
         for (;;)
         {
-            if (a >= n || b >= n)
+            if (a >= nslots || b >= nslots)
                 break;
             const uint32_t a0 = a, b0 = b;
-            uint32_t pa = CachedLoad(&L[a]), pb = CachedLoad(&L[b]);
+            uint64_t ea = CachedLoad64(&E[a]), eb = CachedLoad64(&E[b]);
             uint32_t ha = 0, hb = 0;
             bool bad = false;
-            while (pa != a || pb != b)
+            while (static_cast<uint32_t>(ea) != a || static_cast<uint32_t>(eb) != b)
             {
-                if (pa >= n || pa > a || pb >= n || pb > b)
+                const uint32_t pa = static_cast<uint32_t>(ea), qa = static_cast<uint32_t>(ea >> 32);
+                const uint32_t pb = static_cast<uint32_t>(eb), qb = static_cast<uint32_t>(eb >> 32);
+                if (pa >= nslots || qa > ka || pb >= nslots || qb > kb)
                 {
                     bad = true;
                     break;
@@ -177,36 +199,62 @@ namespace swmi
                 if (pa != a)
                 {
                     a = pa;
-                    pa = CachedLoad(&L[a]);
+                    ka = qa;
+                    ea = CachedLoad64(&E[a]);
                     ++ha;
                 }
                 if (pb != b)
                 {
                     b = pb;
-                    pb = CachedLoad(&L[b]);
+                    kb = qb;
+                    eb = CachedLoad64(&E[b]);
                     ++hb;
                 }
             }
             if (bad)
                 break;
             if (ha > 1)
-                (void)__hip_atomic_fetch_min(&L[a0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                (void)SlotLinkMin(&E[a0], SlotEntry(ka, a));
             if (hb > 1)
-                (void)__hip_atomic_fetch_min(&L[b0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                (void)SlotLinkMin(&E[b0], SlotEntry(kb, b));
             if (a == b)
                 return;
-            if (a > b)
+            if (ka > kb)
             {
-                const uint32_t t = a;
+                const uint32_t t = a, tk = ka;
                 a = b;
+                ka = kb;
                 b = t;
+                kb = tk;
             }
-            const uint32_t old = __hip_atomic_fetch_min(&L[b], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old == b)
+            const uint64_t old = SlotLinkMin(&E[b], SlotEntry(ka, a));
+            if (old == SlotEntry(kb, b))
                 return;
-            b = old;
+            b = static_cast<uint32_t>(old);
+            kb = static_cast<uint32_t>(old >> 32);
         }
         (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // Canonical root (minimum site index) of the cluster of slot c; c_max bounds the slots. A chase that
+    // leaves the slots or meets a parent with a larger key clears ok.
+    __device__ __forceinline__ uint32_t SlotRootKey(gptr<uint64_t> E, uint32_t c, const uint32_t c_max, bool& ok)
+    {
+        uint64_t e = CachedLoad64(&E[c]);
+        uint32_t k = static_cast<uint32_t>(e >> 32);
+        while (static_cast<uint32_t>(e) != c)
+        {
+            const uint32_t p = static_cast<uint32_t>(e), kp = static_cast<uint32_t>(e >> 32);
+            if (p >= c_max || kp > k)
+            {
+                ok = false;
+                break;
+            }
+            c = p;
+            k = kp;
+            e = CachedLoad64(&E[c]);
+        }
+        return static_cast<uint32_t>(e >> 32);
     }
 
     // ---- work decomposition ----
@@ -346,7 +394,7 @@ namespace swmi
         // word of its own in 64-127 (the target of LDS operations a lane has no use for, so that they need
         // no exec mask); narrow tiles keep their union queue here
         __shared__ uint32_t Q[kWavesPerBlock * kQw];
-        __shared__ uint32_t FB[kSites / 32];   // flip bits of a pending flip, by block-local root index
+        __shared__ uint32_t FB[2u * kSites / 32u];   // flip bits of a pending flip, by label index (root or qcap + slot)
         __shared__ uint64_t s_first_B[kWavesPerBlock];
         __shared__ uint64_t AC[8];              // Numerical Recipes pairs (c << 32 | a) by selector
 
@@ -390,7 +438,7 @@ namespace swmi
         const Rsrc rs_spins = MakeRsrc(d.spins + bl.base, span);
         const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
         const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
-        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * twr;
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * d.fbw;
         const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
         const bool last_wave = w + 1u == nwaves;
         const uint32_t sx_r = bl.sx + 1u == d.tiles_x ? 0u : bl.sx + 1u;   // block column to the right
@@ -404,7 +452,7 @@ namespace swmi
         // only under a pending flip) and every candidate for the spin below the tile (which one is used depends
         // on the counters).
         if (!injected)
-            for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
+            for (uint32_t i = threadIdx.x; i < d.fbw; i += 256u)
                 FB[i] = fbits_g[i];
         const uint32_t y_last = (bl.wyb + r_begin + (nrow > 0 ? nrow : 1u) - 1u) * rpw + lrow;
         const uint32_t below = (r_begin + nrow) * bl.stride + lx;   // the site below the tile, inside the block
@@ -475,8 +523,9 @@ namespace swmi
         const uint64_t low = TW64 ? 0ull : ((1ull << (64u - tw)) - 1ull);
         const uint64_t lanes_below = (1ull << lane) - 1ull;
 
-        // pending flip: the block's flip bits (loaded into FB above), one per block-local root index (k_resolve
-        // wrote the bit of every root, edge-reaching or not), so a site's flip is the bit at its label's root
+        // pending flip: the block's flip bits (loaded into FB above), one per label index (k_resolve wrote the
+        // bit of every cluster: by root index, or by qcap + slot for an edge-reaching one), so a site's flip is
+        // the bit at its label's index
         const auto flip_of = [&](const uint32_t lab) { return (LdsLoad(&FB[(lab & 0x7FFFu) >> 5]) >> (lab & 31u)) & 1u; };
 
         // spins below the tile (down halo), needed last but fetched first. Under a pending flip, below the
@@ -846,8 +895,13 @@ namespace swmi
                         *At(Pl, (r_begin + r0 + i) * 256u) = xs[i];
             }
         }
+        // (taller tiles: a flagged entry is a root's own, first flagged by the edge marks, later holding its
+        // slot label; either way the site is its own root)
         const auto root_of = [&](const uint32_t r) {
-            return one_chunk ? xs[r] : (LdsLoad(At(Pl, (r_begin + r) * 256u)) & (kPFlag - 1u));
+            if (one_chunk)
+                return xs[r];
+            const uint32_t v = LdsLoad(At(Pl, (r_begin + r) * 256u));
+            return v >= kPFlag ? (r_begin + r) * 256u + lane4 : v;
         };
         __syncthreads();   // no chase may meet a flagged entry
 
@@ -887,48 +941,68 @@ namespace swmi
         }
         __syncthreads();
 
-        // labels out; the roots of edge-reaching clusters start their global union-find entry and are
-        // listed (per tile) for the resolve pass. The outputs are written through buffer views of this
-        // block's part of each array; a lane with nothing to store gets an offset past the view, and the
-        // hardware drops that store (no exec mask). A tile lists at most min(tile sites, block boundary
-        // sites) roots, which is the list's capacity.
-        const uint32_t cap = d.edge_cap;
-        const Rsrc rs_parent = MakeRsrc(d.parent + bl.base, span * 4u);
-        const Rsrc rs_elist = MakeRsrc(d.edge_list + (kWavesPerBlock * bl.lb + w) * cap, cap * 2u);
-        const Rsrc rs_ecol = MakeRsrc(d.ecol + 2u * bl.lb * d.ecol_stride, 4u * d.ecol_stride);
-        const uint32_t ecol_off = LaneBit(side) ? ((rbound ? d.ecol_stride : 0u) + r_begin * rpw + lrow) * 2u : kNoStore;
+        // Slots: each root of an edge-reaching cluster gets the next slot of its tile (its wave's tile, in
+        // row order), its union-find entry starts as a root keyed by its site, and its own entry in P becomes
+        // kPFlag | (qcap + slot) << 2, so that after the barrier every site of the cluster reads the label
+        // 0x8000 | (qcap + slot). A tile has at most min(tile sites, block boundary sites) edge roots, its
+        // slot capacity; more would be a broken invariant, reported through the error flag.
+        const uint32_t cap = d.edge_cap, qcap = d.qcap;
+        const uint32_t cbase = (kWavesPerBlock * bl.lb + w) * cap;   // this tile's first slot in the lattice
+        const Rsrc rs_cpar = MakeRsrc(d.cpar + cbase, cap * 8u);
         uint32_t nedge = 0;
+        const auto assign = [&](const uint32_t r, const uint32_t root) {
+            const uint32_t wr = r_begin + r;
+            const uint32_t self = wr * 256u + lane4;
+            const uint32_t p = LdsLoad(At(P, root));   // root, with kPFlag if the cluster reaches the edge
+            const uint64_t K = __ballot(p >= kPFlag && root == self);
+            const bool listed = LaneBit(K);
+            const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
+            const uint32_t slot = w * cap + min(pos, cap - 1u);
+            __hip_atomic_store(listed ? At(P, self) : dummy, kPFlag | ((qcap + slot) << 2), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            const u32x2 entry = {cbase + pos, bl.base + wr * bl.stride + lane};   // (site << 32) | slot
+            __builtin_amdgcn_raw_buffer_store_b64(entry, rs_cpar, listed ? pos * 8u : kNoStore, 0, 0);
+            nedge += static_cast<uint32_t>(__popcll(K));
+        };
+        // labels out: the block-local labels, and (label << 16) | root index of the block's first and last
+        // column and first and last wave row for the merge and the halo flips. The outputs go through buffer
+        // views of this block's part of each array; a lane with nothing to store gets an offset past the
+        // view, and the hardware drops that store (no exec mask).
+        const Rsrc rs_ecol = MakeRsrc(d.ecol + 2u * bl.lb * d.ecol_stride, 8u * d.ecol_stride);
+        const uint32_t ecol_off = LaneBit(side) ? ((rbound ? d.ecol_stride : 0u) + r_begin * rpw + lrow) * 4u : kNoStore;
+        const Rsrc rs_erow = MakeRsrc(d.erow + 2u * 64u * bl.lb, 2u * 64u * 4u);
         const auto emit = [&](const uint32_t r, const uint32_t root) {
             const uint32_t wr = r_begin + r;
             const uint32_t rowoff = wr * bl.stride;   // sites from the block's first site
-            const uint32_t p = LdsLoad(At(P, root));   // root, with kPFlag if the cluster reaches the edge
-            const uint32_t lab = p >> 2;
+            const uint32_t lab = LdsLoad(At(P, root)) >> 2;
+            const uint32_t e = (lab << 16) | (root >> 2);
             __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(lab), rs_local, lane * 2u, rowoff * 2u, 0);
-            __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(lab), rs_ecol, ecol_off, r * rpw * 2u, 0);
-            const uint64_t K = __ballot(p >= kPFlag && root == wr * 256u + lane4);
-            const bool listed = LaneBit(K);
-            __builtin_amdgcn_raw_buffer_store_b32(bl.base + rowoff + lane, rs_parent, listed ? lane4 : kNoStore, rowoff * 4u, 0);
-            const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
-            __builtin_amdgcn_raw_buffer_store_b16(static_cast<uint16_t>(wr * 64u + lane), rs_elist, pos * 2u + (listed ? 0u : kNoStore), 0, 0);
-            nedge += static_cast<uint32_t>(__popcll(K));
+            __builtin_amdgcn_raw_buffer_store_b32(e, rs_ecol, ecol_off, r * rpw * 4u, 0);
+            if (wr == 0u)
+                __builtin_amdgcn_raw_buffer_store_b32(e, rs_erow, lane4, 0, 0);
+            if (wr + 1u == bl.nwr)
+                __builtin_amdgcn_raw_buffer_store_b32(e, rs_erow, lane4, 256u, 0);
         };
-        const auto emit_rows = [&](auto full_t) {
+        const auto rows_do = [&](auto full_t, auto&& f) {
             constexpr bool kFull = decltype(full_t)::value;
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
                 if (kFull || i < nrow)
-                    emit(i, xs[i]);
+                    f(i, xs[i]);
         };
-        if (full)
-            emit_rows(std::true_type{});
-        else if (one_chunk)
-            emit_rows(std::false_type{});
-        else
-        {
-            for (uint32_t r = 0; r < nrow; ++r)
-                emit(r, root_of(r));
-        }
+        const auto for_rows = [&](auto&& f) {
+            if (full)
+                rows_do(std::true_type{}, f);
+            else if (one_chunk)
+                rows_do(std::false_type{}, f);
+            else
+                for (uint32_t r = 0; r < nrow; ++r)
+                    f(r, root_of(r));
+        };
+        for_rows(assign);
+        __syncthreads();   // every slot label in place
+        for_rows(emit);
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         if (lane == 0)
         {
@@ -965,26 +1039,33 @@ namespace swmi
     // One two-wave workgroup per block: wavefront 0 takes the right edge (one lane per lattice row),
     // wavefront 1 the down edge (one lane per column). Two waves per block keep every block of a 4096^2
     // lattice resident at once, so the latency-bound union chains of all blocks overlap.
-    // bit `lane` of `bits`: a bond from the site labelled va (this block) to the site labelled vb (block
-    // at base_b)
+    // Bit `lane` of `bits`: a bond from a site of this block to a site of block lb_b (first site base_b), whose
+    // edge entries (ecol / erow: (label << 16) | root index q) are va and vb. Both sites belong to clusters
+    // that reach their block's edge, so their labels name slots.
     __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t va,
-                                              const uint32_t vb, const uint32_t base_b, gptr<uint32_t> err, uint64_t* H)
+                                              const uint32_t vb, const uint32_t lb_b, const uint32_t base_b, gptr<uint32_t> err,
+                                              uint64_t* H)
     {
         const uint32_t lane = threadIdx.x & 63u;
         const bool bit = LaneBit(bits);
-        uint32_t a = 0, bq = 0;
+        const uint32_t slots = kWavesPerBlock * d.edge_cap;
+        uint32_t ka = 0, kb = 0, ca = 0, cb = 0;
         if (bit)
         {
-            if (!(va & kEdgeFlag) || !(vb & kEdgeFlag))
+            const uint32_t la = va >> 16, lb = vb >> 16;
+            const uint32_t sa = (la & 0x7FFFu) - d.qcap, sb = (lb & 0x7FFFu) - d.qcap;
+            if (!(la & kEdgeFlag) || !(lb & kEdgeFlag) || sa >= slots || sb >= slots)
                 (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a = SiteOf(bl.base, bl.stride, va & 0x7FFFu);
-            bq = SiteOf(base_b, bl.stride, vb & 0x7FFFu);
+            ka = SiteOf(bl.base, bl.stride, va & 0xFFFFu);
+            kb = SiteOf(base_b, bl.stride, vb & 0xFFFFu);
+            ca = bl.lb * slots + min(sa, slots - 1u);
+            cb = lb_b * slots + min(sb, slots - 1u);
         }
         // One union per distinct pair of roots (below T_c most bonds of an edge join the same two clusters):
         // the lanes insert their pairs into the wave's LDS hash set H (128 slots for at most 64 pairs, linear
         // probing, so every insert ends), and the lane whose insert lands leads the union for its pair.
-        const uint32_t lo = min(a, bq), hi = max(a, bq);
-        const bool want = bit && a != bq;
+        const uint32_t lo = min(ka, kb), hi = max(ka, kb);
+        const bool want = bit && ka != kb;
         bool leader = false;
         if (__ballot(want))
         {
@@ -1012,7 +1093,7 @@ namespace swmi
             }
         }
         if (leader)
-            GlobalUnion(Glob(d.parent), lo, hi, d.n0 * d.n1, err);
+            SlotUnion(Glob(d.cpar), ca, ka, cb, kb, d.n_blocks * slots, err);
     }
 
     __global__ __launch_bounds__(128) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
@@ -1028,21 +1109,21 @@ namespace swmi
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         const uint32_t lane = threadIdx.x & 63u;
         const uint32_t w = Uniform(threadIdx.x >> 6);
-        const uint32_t n0 = d.n0, n1 = d.n1, tw = d.tw, rpw = d.rpw, twr = d.twr;
+        const uint32_t tw = d.tw, rpw = d.rpw, twr = d.twr;
         const gptr<uint32_t> err = Glob(d.state) + kStErr;
         __shared__ uint64_t HS[2][128];   // per wave: hash set of the root pairs of one edge pass
 
         if (w == 0)
         {
             // right edge: the block's lattice rows, 64 per pass (lane = row); bit r of word 1 + t is row
-            // t * twr * rpw + r; labels from the compact edge columns, loaded with the bond words
+            // t * twr * rpw + r; entries from the compact edge columns, loaded with the bond words
             const uint32_t rows = bl.nwr * rpw, per_tile = twr * rpw;
             const uint32_t sx_r = (bl.sx + 1u == d.tiles_x) ? 0u : bl.sx + 1u;
             const uint32_t lb_r = bl.sy * d.tiles_x + sx_r;
             const uint32_t base_r = BlockBase(d, sx_r, bl.sy);
-            const gptr<const uint16_t> ec = Glob(static_cast<const uint16_t*>(d.ecol));
-            const gptr<const uint16_t> ea = ec + (2u * bl.lb + 1u) * d.ecol_stride;
-            const gptr<const uint16_t> eb = ec + (2u * lb_r) * d.ecol_stride;
+            const gptr<const uint32_t> ec = Glob(static_cast<const uint32_t*>(d.ecol));
+            const gptr<const uint32_t> ea = ec + (2u * bl.lb + 1u) * d.ecol_stride;
+            const gptr<const uint32_t> eb = ec + (2u * lb_r) * d.ecol_stride;
             for (uint32_t p0 = 0; p0 < rows; p0 += 64u)
             {
                 const uint32_t row = min(p0 + lane, rows - 1u);
@@ -1052,21 +1133,20 @@ namespace swmi
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
                 const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
                 if (bits)
-                    MergeEdge(d, bl, bits, va, vb, base_r, err, HS[0]);
+                    MergeEdge(d, bl, bits, va, vb, lb_r, base_r, err, HS[0]);
             }
         }
         else
         {
-            // down edge: lanes of the last lattice row of the block
-            const uint32_t lx = lane & (tw - 1u);
-            const uint32_t y = (bl.wyb + bl.nwr - 1u) * rpw + lane / tw;
-            const uint32_t x = bl.sx * tw + lx;
-            const bool wrap = y + 1u == n1;
-            const gptr<const uint16_t> loc = Glob(static_cast<const uint16_t*>(d.local));
-            const uint32_t va = loc[y * n0 + x], vb = loc[(wrap ? 0u : y + 1u) * n0 + x];
+            // down edge: the lanes of the block's last lattice row (the last lattice row of its last wave row)
+            // against the first lattice row of the block below (lanes 0 .. tw - 1 of its first wave row)
+            const uint32_t sy_b = bl.sy + 1u == d.blocks_y ? 0u : bl.sy + 1u;
+            const uint32_t lb_b = sy_b * d.tiles_x + bl.sx;
+            const gptr<const uint32_t> er = Glob(static_cast<const uint32_t*>(d.erow));
+            const uint32_t va = er[2u * 64u * bl.lb + 64u + lane], vb = er[2u * 64u * lb_b + (lane & (tw - 1u))];
             const uint64_t down_bits = bb[0];
             if (down_bits)
-                MergeEdge(d, bl, down_bits, va, vb, BlockBase(d, bl.sx, wrap ? 0u : bl.sy + 1u), err, HS[1]);
+                MergeEdge(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1]);
         }
     }
 
@@ -1088,10 +1168,11 @@ namespace swmi
         return random_rule ? (Fmix32(RealSite(d, root) ^ key) >> 31) : (root & 1u);
     }
 
-    // A block's flip bitmap has one bit per block-local index q: for the root of a cluster that stays
-    // inside the block, the rule applied to the root's own site; for an edge-reaching root, the rule
-    // applied to its canonical label (set by EdgeRootBits). Word w of the rule part: bit j for q = 32w + j
-    // (block bases and row strides are even, so under the parity rule q's parity is its site's parity).
+    // A block's flip bitmap has one bit per label index: bit q (q < qcap) for the cluster rooted at block-local
+    // index q that stays inside the block, the rule applied to the root's own site; bit qcap + s for the
+    // edge-reaching cluster whose root holds slot s, the rule applied to its canonical root (set by the
+    // resolve). Word w of the rule part: bit j for q = 32w + j (block bases and row strides are even, so under
+    // the parity rule q's parity is its site's parity).
     __device__ __forceinline__ uint32_t RuleWord(const LatDesc& d, const Block& bl, const uint32_t w, const bool random_rule,
                                                  const uint32_t key)
     {
@@ -1103,70 +1184,56 @@ namespace swmi
         return v;
     }
 
-    // flip bit of a site with label lab (root index, with the edge flag) from the block's bitmap
-    __device__ __forceinline__ uint32_t FlipOfLabel(const uint32_t lab, const uint32_t* fb, const uint32_t qmax, bool& ok)
+    // Bitmap index of a label: the root index, or qcap + slot for an edge-reaching cluster. An index outside
+    // the block's range (a broken invariant) clears ok and reads bit 0.
+    __device__ __forceinline__ uint32_t LabelBit(const uint32_t lab, const uint32_t qmax, const uint32_t qcap, const uint32_t slots,
+                                                 bool& ok)
     {
-        uint32_t q = lab & 0x7FFFu;
-        if (q >= qmax)
-        {
+        const uint32_t i = lab & 0x7FFFu;
+        const bool good = (lab & kEdgeFlag) ? (i >= qcap && i < qcap + slots) : (i < qmax);
+        if (!good)
             ok = false;
-            q = 0;
-        }
-        return (fb[q >> 5] >> (q & 31u)) & 1u;
+        return good ? i : 0u;
     }
 
-    // Phase 1 of the resolve: the roots of the block's edge-reaching clusters (the tile lists written by
-    // k_bond_label) are chased in the global parent array, one root per thread, and fb's bit at the root's
-    // block-local index (the rule applied to the root's own site, set by the caller) is turned into the rule
-    // applied to the canonical label found. Each root is listed once.
-    template <uint32_t kThreads>
-    __device__ __forceinline__ void EdgeRootBits(const LatDesc& d, const Block& bl, uint32_t* fb, const bool random_rule,
-                                                 const uint32_t key, const uint32_t qmax, bool& ok)
+    // flip bit of a site with label lab from the block's bitmap
+    __device__ __forceinline__ uint32_t FlipOfLabel(const uint32_t lab, const uint32_t* fb, const uint32_t qmax, const uint32_t qcap,
+                                                    const uint32_t slots, bool& ok)
     {
-        const gptr<uint32_t> L = Glob(d.parent);
+        const uint32_t i = LabelBit(lab, qmax, qcap, slots, ok);
+        return (fb[i >> 5] >> (i & 31u)) & 1u;
+    }
+
+    // The block's edge-root slots (per tile, the dense prefix k_bond_label filled), for i in
+    // [first, total) step `stride`: f(slot within the block, lattice-wide slot).
+    template <typename F>
+    __device__ __forceinline__ void ForBlockSlots(const LatDesc& d, const Block& bl, const uint32_t first, const uint32_t stride, F&& f)
+    {
         const uint32_t cap = d.edge_cap;
         const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
         const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
         const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
-        const gptr<uint16_t> elist = Glob(d.edge_list) + kWavesPerBlock * bl.lb * cap;
-        for (uint32_t i = threadIdx.x; i < total; i += kThreads)
+        for (uint32_t i = first; i < total; i += stride)
         {
-            const uint32_t j = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
-            const uint32_t first = j == 0u ? 0u : (j == 1u ? e1 : (j == 2u ? e2 : e3));
-            const uint32_t q = elist[j * cap + (i - first)];
-            if (q >= qmax)
-            {
-                ok = false;
-                continue;
-            }
-            uint32_t x = SiteOf(bl.base, bl.stride, q);
-            uint32_t p = CachedLoad(&L[x]);
-            while (p != x)
-            {
-                if (p > x)
-                {
-                    ok = false;
-                    break;
-                }
-                x = p;
-                p = CachedLoad(&L[x]);
-            }
-            const uint32_t own = FlipBit(d, SiteOf(bl.base, bl.stride, q), random_rule, key);
-            if (FlipBit(d, x, random_rule, key) != own)
-                (void)__hip_atomic_fetch_xor(&fb[q >> 5], 1u << (q & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t t = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
+            const uint32_t start = t == 0u ? 0u : (t == 1u ? e1 : (t == 2u ? e2 : e3));
+            const uint32_t slot = t * cap + (i - start);
+            f(slot, kWavesPerBlock * cap * bl.lb + slot);
         }
     }
 
-    // End of a sweep, per block (two waves): phase 1 above, then the block's flip bits and the flipped
-    // spins of its first column and first wave row (the halos its left and upper neighbours read in the
-    // next sweep) are written out, and the lattice's counters move on: applied = the iteration whose
-    // spins the array holds, iteration += 2. The spin array itself is left for the next sweep to flip.
+    // End of a sweep, per block (two waves): the slots of the block's edge-reaching clusters are chased in the
+    // slot union-find, and each slot's bit becomes the flip rule applied to its canonical root; then the
+    // block's flip bitmap and the flipped spins of its first column and first wave row (the halos its left
+    // and upper neighbours read in the next sweep) are written out, and the lattice's counters move on:
+    // applied = the iteration whose spins the array holds, iteration += 2. The spin array itself is left for
+    // the next sweep to flip.
     __global__ __launch_bounds__(128) void k_resolve(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                      const uint32_t total_blocks)
     {
         constexpr uint32_t kThreads = 128;
         constexpr uint32_t kMaxSites = kWavesPerBlock * 64u * 64u;
-        __shared__ uint32_t fb[kMaxSites / 32];
+        __shared__ uint32_t fb[2u * kMaxSites / 32u];
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
@@ -1175,11 +1242,11 @@ namespace swmi
         if (stp[kStActive] == 0u)
             return;
         const Block bl = DecodeBlock(d, b);
-        const uint32_t qmax = bl.nwr * 64u;
+        const uint32_t qmax = bl.nwr * 64u, qcap = d.qcap, slots = kWavesPerBlock * d.edge_cap;
         const bool random_rule = d.flip_rule != 0u;
         const uint32_t key = random_rule ? stp[kStFlipKey] : 0u;
-        const gptr<const uint16_t> local = Glob(static_cast<const uint16_t*>(d.local));
-        const gptr<const uint16_t> ecol0 = Glob(static_cast<const uint16_t*>(d.ecol)) + 2u * bl.lb * d.ecol_stride;
+        const gptr<const uint32_t> erow0 = Glob(static_cast<const uint32_t*>(d.erow)) + 2u * 64u * bl.lb;
+        const gptr<const uint32_t> ecol0 = Glob(static_cast<const uint32_t*>(d.ecol)) + 2u * bl.lb * d.ecol_stride;
         const gptr<uint8_t> halo = Glob(d.halo) + (stp[kStHaloSide] * d.n_blocks + bl.lb) * d.halo_stride;
         bool ok = true;
 
@@ -1199,22 +1266,31 @@ namespace swmi
             if (t < rows)
             {
                 hs[k] = halo[t];
-                hl[k] = ecol0[t];
+                hl[k] = ecol0[t] >> 16;
             }
             else if (t < rows + 64u)
             {
                 hs[k] = halo[d.ecol_stride + (t - rows)];
-                hl[k] = local[bl.base + (t - rows)];
+                hl[k] = erow0[t - rows] >> 16;
             }
         }
-        for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
-            fb[i] = RuleWord(d, bl, i, random_rule, key);
+        for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
+            fb[i] = i < qmax / 32u ? RuleWord(d, bl, i, random_rule, key) : 0u;
         __syncthreads();
-        EdgeRootBits<kThreads>(d, bl, fb, random_rule, key, qmax, ok);
+        // each slot's canonical root, one slot per thread; its bit starts clear (edge-reaching clusters have
+        // their own bits) and is set where the rule flips the root
+        const gptr<uint64_t> E = Glob(d.cpar);
+        const uint32_t c_max = d.n_blocks * slots;
+        ForBlockSlots(d, bl, threadIdx.x, kThreads, [&](const uint32_t slot, const uint32_t c) {
+            const uint32_t root = SlotRootKey(E, c, c_max, ok);
+            if (FlipBit(d, root, random_rule, key))
+                (void)__hip_atomic_fetch_or(&fb[(qcap + slot) >> 5], 1u << ((qcap + slot) & 31u), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+        });
         __syncthreads();
 
-        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * 8u * d.twr;
-        for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
+        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * d.fbw;
+        for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
             fbits_g[i] = fb[i];
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k)
@@ -1222,7 +1298,7 @@ namespace swmi
             const uint32_t t = threadIdx.x + k * kThreads;
             if (t < rows + 64u)
             {
-                const uint32_t v = (hs[k] ^ FlipOfLabel(hl[k], fb, qmax, ok)) & 1u;
+                const uint32_t v = (hs[k] ^ FlipOfLabel(hl[k], fb, qmax, qcap, slots, ok)) & 1u;
                 halo[t < rows ? t : d.ecol_stride + (t - rows)] = static_cast<uint8_t>(v);
             }
         }
@@ -1246,7 +1322,7 @@ namespace swmi
         constexpr uint32_t kPass = kThreads / 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
         constexpr uint32_t kCh = 8;                 // passes loaded together
         constexpr uint32_t kMaxSites = kWavesPerBlock * 64u * 64u;
-        __shared__ uint32_t fb[kMaxSites / 32];
+        __shared__ uint32_t fb[2u * kMaxSites / 32u];
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
@@ -1257,7 +1333,7 @@ namespace swmi
         const Block bl = DecodeBlock(d, b);
         const gptr<uint16_t> local = Glob(d.local);
         const gptr<uint8_t> spins = Glob(d.spins);
-        const uint32_t qmax = bl.nwr * 64u;
+        const uint32_t qmax = bl.nwr * 64u, qcap = d.qcap, slots = kWavesPerBlock * d.edge_cap;
         const uint32_t sub = threadIdx.x >> 4;
         const uint32_t col = (threadIdx.x & 15u) * 4u;
         bool ok = true;
@@ -1280,8 +1356,8 @@ namespace swmi
             }
         };
         load(0u);
-        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * 8u * d.twr;
-        for (uint32_t i = threadIdx.x; i < qmax / 32u; i += kThreads)
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * d.fbw;
+        for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
             fb[i] = fbits_g[i];
         __syncthreads();
 
@@ -1298,7 +1374,7 @@ namespace swmi
                     uint32_t flip = 0;
 #pragma unroll
                     for (uint32_t e = 0; e < 4; ++e)
-                        flip |= FlipOfLabel(lab[e], fb, qmax, ok) << (8u * e);
+                        flip |= FlipOfLabel(lab[e], fb, qmax, qcap, slots, ok) << (8u * e);
                     *(gptr<uint32_t>)(spins + site) = sv[k] ^ flip;
                 }
             }
@@ -1328,14 +1404,14 @@ namespace swmi
         state[kStApplied] = state[kStIteration];
     }
 
-    // Inspection kernel: the canonical label (minimum site index) of every site, written to parent[]. The
-    // same phase 1 with the canonical label itself kept in LDS (one word per block site).
+    // Inspection kernel: the canonical label (minimum site index) of every site, written to parent[]: the
+    // canonical root of every slot of the block (kept in LDS, one word per slot), then every site's label.
     __global__ __launch_bounds__(256) void k_write_labels(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                           const uint32_t total_blocks)
     {
         constexpr uint32_t kPass = 16;   // wave rows per pass of the workgroup (16 lanes per wave row)
         constexpr uint32_t kCh = 4;      // passes loaded together
-        extern __shared__ uint32_t R[];  // kWavesPerBlock * twr * 64 entries
+        extern __shared__ uint32_t R[];  // kWavesPerBlock * edge_cap entries (at most the block's sites)
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
@@ -1343,7 +1419,7 @@ namespace swmi
         const Block bl = DecodeBlock(d, b);
         const gptr<uint32_t> L = Glob(d.parent);
         const gptr<uint16_t> local = Glob(d.local);
-        const uint32_t qmax = bl.nwr * 64u;
+        const uint32_t qmax = bl.nwr * 64u, qcap = d.qcap, slots = kWavesPerBlock * d.edge_cap;
         const uint32_t sub = threadIdx.x >> 4;
         const uint32_t col = (threadIdx.x & 15u) * 4u;
         bool ok = true;
@@ -1361,38 +1437,10 @@ namespace swmi
         };
         load(0u);
 
-        // phase 1: canonical labels of the edge-reaching roots
-        {
-            const uint32_t cap = d.edge_cap;
-            const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
-            const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
-            const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
-            const gptr<uint16_t> elist = Glob(d.edge_list) + kWavesPerBlock * bl.lb * cap;
-            for (uint32_t i = threadIdx.x; i < total; i += 256u)
-            {
-                const uint32_t j = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
-                const uint32_t first = j == 0u ? 0u : (j == 1u ? e1 : (j == 2u ? e2 : e3));
-                const uint32_t q = elist[j * cap + (i - first)];
-                if (q >= qmax)
-                {
-                    ok = false;
-                    continue;
-                }
-                uint32_t x = SiteOf(bl.base, bl.stride, q);
-                uint32_t p = CachedLoad(&L[x]);
-                while (p != x)
-                {
-                    if (p > x)
-                    {
-                        ok = false;
-                        break;
-                    }
-                    x = p;
-                    p = CachedLoad(&L[x]);
-                }
-                R[q] = x;
-            }
-        }
+        // phase 1: canonical root of every slot in use
+        const gptr<uint64_t> E = Glob(d.cpar);
+        const uint32_t c_max = d.n_blocks * slots;
+        ForBlockSlots(d, bl, threadIdx.x, 256u, [&](const uint32_t slot, const uint32_t c) { R[slot] = SlotRootKey(E, c, c_max, ok); });
         __syncthreads();
 
         // phase 2: every site's label, four sites per lane
@@ -1410,15 +1458,8 @@ namespace swmi
 #pragma unroll
                     for (uint32_t e = 0; e < 4; ++e)
                     {
-                        uint32_t q = lab[e] & 0x7FFFu;
-                        const bool edge = (lab[e] & kEdgeFlag) != 0u;
-                        if (q >= qmax)
-                        {
-                            ok = false;
-                            q = 0;
-                        }
-                        const uint32_t g = R[edge ? q : 0u];
-                        root[e] = edge ? g : SiteOf(bl.base, bl.stride, q);
+                        const uint32_t i = LabelBit(lab[e], qmax, qcap, slots, ok);
+                        root[e] = (lab[e] & kEdgeFlag) ? R[i >= qcap ? i - qcap : 0u] : SiteOf(bl.base, bl.stride, i);
                     }
                     *(gptr<u32x4>)(L + site) = u32x4{root[0], root[1], root[2], root[3]};
                 }

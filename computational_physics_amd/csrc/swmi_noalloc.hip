@@ -3,12 +3,11 @@
 // hipMalloc).
 //
 // The caller passes the bond field and the label array in device memory. Every other buffer the kernels
-// need (block-local labels, block-edge bond bits, edge-root lists, edge columns, per-lattice counters and
-// the lattice descriptor) is carved out of one file-scope device array, `scratch`,
-// which every GPU holds for the life of the process. For a lattice that needs no row padding the global
-// union-find runs in the caller's label array itself, which the last kernel overwrites with the canonical
-// labels; a padded lattice gets its bond field padded into scratch and its union-find in scratch, and a
-// last kernel writes the unpadded labels. The call enqueues kernels on the caller's stream and returns: no
+// need (block-local labels, the union-find over edge-root slots, block-edge bond bits, slot counts, edge
+// columns and rows, per-lattice counters and the lattice descriptor) is carved out of one file-scope device array, `scratch`,
+// which every GPU holds for the life of the process. For a lattice that needs no row padding the last kernel
+// writes the canonical labels straight into the caller's label array; a padded lattice gets its bond field
+// padded into scratch and its labels written into scratch, and a last kernel writes the unpadded labels. The call enqueues kernels on the caller's stream and returns: no
 // allocation, no memset, no host synchronisation.
 //
 // Every call on a GPU shares that GPU's copy of the array, so two calls that overlap on one GPU (two
@@ -86,10 +85,11 @@ namespace swmi
             *d_desc = c.Take<LatDesc>(1);
             d.state = c.Take<uint32_t>(kLatStateWords);
             d.local = c.Take<uint16_t>(n);
+            d.cpar = c.Take<uint64_t>(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
             d.bbits = c.Take<uint64_t>(static_cast<size_t>(kBbitWords) * g.n_blocks);
-            d.edge_list = c.Take<uint16_t>(static_cast<size_t>(kWavesPerBlock) * g.n_blocks * g.edge_cap);
             d.edge_count = c.Take<uint32_t>(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
-            d.ecol = c.Take<uint16_t>(2ull * g.ecol_stride * g.n_blocks);
+            d.ecol = c.Take<uint32_t>(2ull * g.ecol_stride * g.n_blocks);
+            d.erow = c.Take<uint32_t>(2ull * 64 * g.n_blocks);
             d.parent = pad ? c.Take<uint32_t>(n) : labels;
             d.bond_in = pad ? c.Take<uint8_t>(n) : bonds;
             d.spins = const_cast<uint8_t*>(d.bond_in);
