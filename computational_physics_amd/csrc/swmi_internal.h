@@ -15,11 +15,12 @@
 //   local    uint16[n]       block label of each site: the block-local index q of its cluster's root, or for a
 //                            cluster that reaches the block edge 0x8000 | (qcap + s), s = the root's slot
 //   cpar     uint64[4*edge_cap*blocks]  global union-find over the roots of edge-reaching clusters, one
-//                            entry per slot: slot s of tile t of block b is c = (4b + t) * edge_cap + s', its
-//                            entry (key << 32) | parent slot, key = the root's site index (the order of the
+//                            entry per slot: slot s of block b is c = 4*edge_cap*b + s, its entry
+//                            (key << 32) | parent slot, key = the root's site index (the order of the
 //                            union-find: a parent's key never exceeds its child's, so a tree's root is the
-//                            minimum site of the component). Each tile numbers its edge roots densely
-//   edge_count uint32[4*blocks]  slots in use in each tile
+//                            minimum site of the component). Each block numbers its edge roots densely, in
+//                            the order their first edge marks land
+//   edge_count uint32[4*blocks]  word 4b: slots in use in block b
 //   bbits    uint64[5*blocks] block-edge bond bits: down edge, then right edge (64 rows per word)
 //   ecol     uint32[2*ecol_stride*blocks] (label << 16) | root index q of each block's first (side 0) and
 //                            last (side 1) column, one entry per lattice row of the block, for the merge
@@ -77,7 +78,7 @@ namespace swmi
         uint8_t* bond_dump;      // per-site bond bytes of the last sweep (nullptr: not recorded)
         const uint8_t* bond_in;  // injected bond bytes (label-only mode), nullptr otherwise
         int32_t* partials;       // 2 int32 (E, M) per tile for observables
-        uint32_t* edge_count;    // kWavesPerBlock counts per block
+        uint32_t* edge_count;    // slots in use, word kWavesPerBlock * block
         uint32_t* ecol;          // block edge columns: [block][side][lattice row in block] of (label << 16) | q
         uint32_t* erow;          // block edge wave rows: [block][side][lane] of (label << 16) | q
         uint32_t ecol_stride;    // lattice rows per block: kWavesPerBlock * twr * rpw

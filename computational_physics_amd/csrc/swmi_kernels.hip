@@ -124,6 +124,18 @@ namespace swmi
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
+    // An empty asm statement that reads v: v's load completes before this point, so it cannot be sunk past a
+    // later branch (an early exit) and is waited for together with the loads issued beside it.
+    __device__ __forceinline__ void Issued(const uint32_t v)
+    {
+        asm volatile("" ::"v"(v));
+    }
+
+    __device__ __forceinline__ void Issued(const uint64_t v)
+    {
+        asm volatile("" ::"v"(v));
+    }
+
     __device__ __forceinline__ void LdsMin(uint32_t* p, const uint32_t v)
     {
         (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -236,11 +248,10 @@ namespace swmi
         (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
-    // Canonical root (minimum site index) of the cluster of slot c; c_max bounds the slots. A chase that
-    // leaves the slots or meets a parent with a larger key clears ok.
-    __device__ __forceinline__ uint32_t SlotRootKey(gptr<uint64_t> E, uint32_t c, const uint32_t c_max, bool& ok)
+    // Key of the canonical root (the minimum site) of the cluster of slot c, whose entry e the caller loaded;
+    // c_max bounds the slots. A chase that leaves the slots or meets a parent with a larger key clears ok.
+    __device__ __forceinline__ uint32_t SlotRootKey(gptr<uint64_t> E, uint32_t c, uint64_t e, const uint32_t c_max, bool& ok)
     {
-        uint64_t e = CachedLoad64(&E[c]);
         uint32_t k = static_cast<uint32_t>(e >> 32);
         while (static_cast<uint32_t>(e) != c)
         {
@@ -397,6 +408,7 @@ namespace swmi
         __shared__ uint32_t FB[2u * kSites / 32u];   // flip bits of a pending flip, by label index (root or qcap + slot)
         __shared__ uint64_t s_first_B[kWavesPerBlock];
         __shared__ uint64_t AC[8];              // Numerical Recipes pairs (c << 32 | a) by selector
+        __shared__ uint32_t s_nslot;            // slots given to the block's edge-reaching clusters
 
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
@@ -407,6 +419,8 @@ namespace swmi
             ParamsOf(threadIdx.x, a, c);
             AC[threadIdx.x] = (static_cast<uint64_t>(c) << 32) | a;
         }
+        if (threadIdx.x == 0)
+            s_nslot = 0u;
         const LatDesc& d = descs[LatticeOf(block_lat, b)];
         const Block bl = DecodeBlock(d, b);
         const uint32_t lane = threadIdx.x & 63u;
@@ -905,66 +919,77 @@ namespace swmi
         };
         __syncthreads();   // no chase may meet a flagged entry
 
-        // sites on the block edge flag their root (lanes off the edge flag their own word of Q). The edge
-        // lanes of a row are `side`, plus every lane of the block's first and last lattice rows.
+        // Sites on the block edge flag their root (lanes off the edge flag their own word of Q). The edge
+        // lanes of a row are `side`, plus every lane of the block's first and last lattice rows. The marks of
+        // up to kRows rows are issued together and return the root's entry as it was: the lane that found it
+        // unflagged (the cluster's first mark) gives the cluster the next slot of the block (one LDS counter,
+        // one addition per wave and chunk of rows), writes the slot's union-find entry (a root keyed by the
+        // root's site) and makes the root's entry in P kPFlag | (qcap + slot) << 2, so that after the barrier
+        // every site of the cluster reads the label 0x8000 | (qcap + slot). A block has at most as many edge
+        // roots as boundary sites, within its slot capacity; more would be a broken invariant, reported
+        // through the error flag.
         const uint64_t side = TW64 ? (PAD ? (1ull | (1ull << elane)) : 0x8000000000000001ull) : __ballot(lx == 0 || rbound);
         const uint64_t top_first = r_begin == 0 ? (TW64 ? vmask : (__ballot(lrow == 0) & vmask)) : 0ull;
         const uint64_t bottom_last = last_wave ? (TW64 ? vmask : (__ballot(lrow == rpw - 1u) & vmask)) : 0ull;
+        const uint32_t qcap = d.qcap, bslots = kWavesPerBlock * d.edge_cap;
+        const Rsrc rs_cpar = MakeRsrc(d.cpar + bslots * bl.lb, bslots * 8u);
         // full tiles (the rule) take the row count and the first/last row masks at compile time
         const bool full = TWR_MAX <= kRows && nrow == kRows;
-        const auto mark_rows = [&](auto full_t) {
+        const auto mark_rows = [&](auto full_t, const uint32_t r0) {
             constexpr bool kFull = decltype(full_t)::value;
+            uint32_t old[kRows], root[kRows];
 #pragma unroll
             for (uint32_t i = 0; i < kRows; ++i)
             {
-                if (kFull || i < nrow)
+                old[i] = kPFlag;
+                root[i] = 0u;
+                const uint32_t r = r0 + i;
+                if (kFull || r < nrow)
                 {
-                    const bool last = kFull ? i + 1u == kRows : i + 1u == nrow;
-                    const uint64_t m = side | (i == 0 ? top_first : 0ull) | (last ? bottom_last : 0ull);
-                    (void)__hip_atomic_fetch_or(LaneBit(m) ? At(P, xs[i]) : dummy, kPFlag, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const bool last = kFull ? i + 1u == kRows : r + 1u == nrow;
+                    const uint64_t m = side | (r == 0u ? top_first : 0ull) | (last ? bottom_last : 0ull);
+                    root[i] = root_of(r);
+                    const uint32_t o = __hip_atomic_fetch_or(LaneBit(m) ? At(P, root[i]) : dummy, kPFlag, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                    old[i] = LaneBit(m) ? o : kPFlag;
                 }
+            }
+            uint64_t F[kRows];
+            uint32_t total = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kRows; ++i)
+            {
+                F[i] = __ballot(old[i] < kPFlag);
+                total += static_cast<uint32_t>(__popcll(F[i]));
+            }
+            if (total == 0u)
+                return;
+            uint32_t base = 0;
+            if (lane == 0u)
+                base = __hip_atomic_fetch_add(&s_nslot, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            base = __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+            for (uint32_t i = 0; i < kRows; ++i)
+            {
+                const bool first = LaneBit(F[i]);
+                const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(F[i] >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(F[i]), 0u));
+                __hip_atomic_store(first ? At(P, root[i]) : dummy, kPFlag | ((qcap + min(pos, bslots - 1u)) << 2),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const u32x2 entry = {bslots * bl.lb + pos, SiteOf(bl.base, bl.stride, root[i] >> 2)};   // (site << 32) | slot
+                __builtin_amdgcn_raw_buffer_store_b64(entry, rs_cpar, first ? pos * 8u : kNoStore, 0, 0);
+                base += static_cast<uint32_t>(__popcll(F[i]));
             }
         };
         if (full)
-            mark_rows(std::true_type{});
+            mark_rows(std::true_type{}, 0u);
         else if (one_chunk)
-            mark_rows(std::false_type{});
+            mark_rows(std::false_type{}, 0u);
         else
-        {
-            for (uint32_t r = 0; r < nrow; ++r)
-            {
-                const uint64_t m = side | (r == 0 ? top_first : 0ull) | (r + 1u == nrow ? bottom_last : 0ull);
-                (void)__hip_atomic_fetch_or(LaneBit(m) ? At(P, root_of(r)) : dummy, kPFlag, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-        __syncthreads();
+            for (uint32_t r0 = 0; r0 < nrow; r0 += kRows)
+                mark_rows(std::false_type{}, r0);
+        __syncthreads();   // every slot label in place
 
-        // Slots: each root of an edge-reaching cluster gets the next slot of its tile (its wave's tile, in
-        // row order), its union-find entry starts as a root keyed by its site, and its own entry in P becomes
-        // kPFlag | (qcap + slot) << 2, so that after the barrier every site of the cluster reads the label
-        // 0x8000 | (qcap + slot). A tile has at most min(tile sites, block boundary sites) edge roots, its
-        // slot capacity; more would be a broken invariant, reported through the error flag.
-        const uint32_t cap = d.edge_cap, qcap = d.qcap;
-        const uint32_t cbase = (kWavesPerBlock * bl.lb + w) * cap;   // this tile's first slot in the lattice
-        const Rsrc rs_cpar = MakeRsrc(d.cpar + cbase, cap * 8u);
-        uint32_t nedge = 0;
-        const auto assign = [&](const uint32_t r, const uint32_t root) {
-            const uint32_t wr = r_begin + r;
-            const uint32_t self = wr * 256u + lane4;
-            const uint32_t p = LdsLoad(At(P, root));   // root, with kPFlag if the cluster reaches the edge
-            const uint64_t K = __ballot(p >= kPFlag && root == self);
-            const bool listed = LaneBit(K);
-            const uint32_t pos = nedge + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(K >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(K), 0u));
-            const uint32_t slot = w * cap + min(pos, cap - 1u);
-            __hip_atomic_store(listed ? At(P, self) : dummy, kPFlag | ((qcap + slot) << 2), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-            const u32x2 entry = {cbase + pos, bl.base + wr * bl.stride + lane};   // (site << 32) | slot
-            __builtin_amdgcn_raw_buffer_store_b64(entry, rs_cpar, listed ? pos * 8u : kNoStore, 0, 0);
-            nedge += static_cast<uint32_t>(__popcll(K));
-        };
         // labels out: the block-local labels, and (label << 16) | root index of the block's first and last
         // column and first and last wave row for the merge and the halo flips. The outputs go through buffer
         // views of this block's part of each array; a lane with nothing to store gets an offset past the
@@ -1000,8 +1025,6 @@ namespace swmi
                 for (uint32_t r = 0; r < nrow; ++r)
                     f(r, root_of(r));
         };
-        for_rows(assign);
-        __syncthreads();   // every slot label in place
         for_rows(emit);
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         if (lane == 0)
@@ -1009,8 +1032,12 @@ namespace swmi
             if (last_wave)
                 bb[0] = down_bits;
             bb[1u + w] = right_bits;
-            Glob(d.edge_count)[kWavesPerBlock * bl.lb + w] = min(nedge, cap);
-            if (nedge > cap)
+        }
+        if (threadIdx.x == 0)
+        {
+            const uint32_t n = LdsLoad(&s_nslot);
+            Glob(d.edge_count)[kWavesPerBlock * bl.lb] = min(n, bslots);
+            if (n > bslots)
                 (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -1103,8 +1130,8 @@ namespace swmi
         if (b >= total_blocks)
             return;
         const LatDesc& d = descs[LatticeOf(block_lat, b)];
-        if (Glob(d.state)[kStActive] == 0u)
-            return;
+        // the lattice's active flag is loaded with the edge data (an inactive lattice links nothing)
+        const gptr<const uint32_t> active = Glob(static_cast<const uint32_t*>(d.state)) + kStActive;
         const Block bl = DecodeBlock(d, b);
         const gptr<uint64_t> bb = Glob(d.bbits) + kBbitWords * bl.lb;
         const uint32_t lane = threadIdx.x & 63u;
@@ -1124,15 +1151,22 @@ namespace swmi
             const gptr<const uint32_t> ec = Glob(static_cast<const uint32_t*>(d.ecol));
             const gptr<const uint32_t> ea = ec + (2u * bl.lb + 1u) * d.ecol_stride;
             const gptr<const uint32_t> eb = ec + (2u * lb_r) * d.ecol_stride;
+            const uint64_t r0 = bb[1], r1 = bb[2], r2 = bb[3], r3 = bb[4];
             for (uint32_t p0 = 0; p0 < rows; p0 += 64u)
             {
                 const uint32_t row = min(p0 + lane, rows - 1u);
                 const uint32_t va = ea[row], vb = eb[row];
-                const uint64_t r0 = bb[1], r1 = bb[2], r2 = bb[3], r3 = bb[4];
+                const uint32_t act = *active;
+                Issued(r0);
+                Issued(r1);
+                Issued(r2);
+                Issued(r3);
+                Issued(va);
+                Issued(vb);
                 const uint32_t t = row / per_tile, r = row - t * per_tile;
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
                 const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
-                if (bits)
+                if (bits && act != 0u)
                     MergeEdge(d, bl, bits, va, vb, lb_r, base_r, err, HS[0]);
             }
         }
@@ -1145,7 +1179,10 @@ namespace swmi
             const gptr<const uint32_t> er = Glob(static_cast<const uint32_t*>(d.erow));
             const uint32_t va = er[2u * 64u * bl.lb + 64u + lane], vb = er[2u * 64u * lb_b + (lane & (tw - 1u))];
             const uint64_t down_bits = bb[0];
-            if (down_bits)
+            const uint32_t act = *active;
+            Issued(va);
+            Issued(vb);
+            if (down_bits && act != 0u)
                 MergeEdge(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1]);
         }
     }
@@ -1204,22 +1241,15 @@ namespace swmi
         return (fb[i >> 5] >> (i & 31u)) & 1u;
     }
 
-    // The block's edge-root slots (per tile, the dense prefix k_bond_label filled), for i in
-    // [first, total) step `stride`: f(slot within the block, lattice-wide slot).
+    // The block's edge-root slots (the dense prefix k_bond_label filled), for i in [first, total) step
+    // `stride`: f(slot within the block, lattice-wide slot).
     template <typename F>
     __device__ __forceinline__ void ForBlockSlots(const LatDesc& d, const Block& bl, const uint32_t first, const uint32_t stride, F&& f)
     {
-        const uint32_t cap = d.edge_cap;
-        const gptr<uint32_t> cnt = Glob(d.edge_count) + kWavesPerBlock * bl.lb;
-        const uint32_t c0 = min(cnt[0], cap), c1 = min(cnt[1], cap), c2 = min(cnt[2], cap), c3 = min(cnt[3], cap);
-        const uint32_t e1 = c0, e2 = e1 + c1, e3 = e2 + c2, total = e3 + c3;
+        const uint32_t slots = kWavesPerBlock * d.edge_cap;
+        const uint32_t total = min(Glob(d.edge_count)[kWavesPerBlock * bl.lb], slots);
         for (uint32_t i = first; i < total; i += stride)
-        {
-            const uint32_t t = (i >= e1 ? 1u : 0u) + (i >= e2 ? 1u : 0u) + (i >= e3 ? 1u : 0u);
-            const uint32_t start = t == 0u ? 0u : (t == 1u ? e1 : (t == 2u ? e2 : e3));
-            const uint32_t slot = t * cap + (i - start);
-            f(slot, kWavesPerBlock * cap * bl.lb + slot);
-        }
+            f(i, slots * bl.lb + i);
     }
 
     // End of a sweep, per block (two waves): the slots of the block's edge-reaching clusters are chased in the
@@ -1239,54 +1269,62 @@ namespace swmi
             return;
         const LatDesc& d = descs[LatticeOf(block_lat, b)];
         const gptr<uint32_t> stp = Glob(d.state);
-        if (stp[kStActive] == 0u)
-            return;
         const Block bl = DecodeBlock(d, b);
         const uint32_t qmax = bl.nwr * 64u, qcap = d.qcap, slots = kWavesPerBlock * d.edge_cap;
-        const bool random_rule = d.flip_rule != 0u;
-        const uint32_t key = random_rule ? stp[kStFlipKey] : 0u;
         const gptr<const uint32_t> erow0 = Glob(static_cast<const uint32_t*>(d.erow)) + 2u * 64u * bl.lb;
         const gptr<const uint32_t> ecol0 = Glob(static_cast<const uint32_t*>(d.ecol)) + 2u * bl.lb * d.ecol_stride;
-        const gptr<uint8_t> halo = Glob(d.halo) + (stp[kStHaloSide] * d.n_blocks + bl.lb) * d.halo_stride;
         bool ok = true;
 
         // halo sites: entry t < rows is the first column at lattice row t of the block, entry rows + j the
-        // first wave row at lane j. Their spins (written into the halo by this sweep's k_bond_label) and
-        // labels are fetched before the chases; the halo is flipped in place.
+        // first wave row at lane j. Their labels and the block's slot count need only the descriptor and are
+        // loaded with the lattice's counters; their spins (written into the halo by this sweep's
+        // k_bond_label) follow with the first entries of the slot chases. The halo is flipped in place.
         const uint32_t rows = bl.nwr * d.rpw;
         // rows <= kWavesPerBlock * 64 (a tile has twr <= tw wave rows of rpw = 64 / tw lattice rows)
         constexpr uint32_t kHaloPer = (kWavesPerBlock * 64u + 64u + kThreads - 1u) / kThreads;
-        uint32_t hs[kHaloPer], hl[kHaloPer];
+        // (straight-line loads from a selected address, every one in range, so that they are all in flight
+        // at once; hoff is the entry's offset in the halo)
+        uint32_t hs[kHaloPer], hl[kHaloPer], hoff[kHaloPer];
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k)
         {
             const uint32_t t = threadIdx.x + k * kThreads;
-            hs[k] = 0;
-            hl[k] = 0;
-            if (t < rows)
-            {
-                hs[k] = halo[t];
-                hl[k] = ecol0[t] >> 16;
-            }
-            else if (t < rows + 64u)
-            {
-                hs[k] = halo[d.ecol_stride + (t - rows)];
-                hl[k] = erow0[t - rows] >> 16;
-            }
+            const bool col = t < rows;
+            const uint32_t j = min(t - rows, 63u);
+            hoff[k] = col ? t : d.ecol_stride + j;
+            hl[k] = *(col ? ecol0 + t : erow0 + j) >> 16;
         }
+        const uint32_t nslot = min(Glob(d.edge_count)[kWavesPerBlock * bl.lb], slots);
+        const uint32_t active = stp[kStActive], hside = stp[kStHaloSide], fkey = stp[kStFlipKey];
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k)
+            Issued(hl[k]);
+        Issued(nslot);
+        Issued(hside);
+        Issued(fkey);
+        if (active == 0u)
+            return;
+        const bool random_rule = d.flip_rule != 0u;
+        const uint32_t key = random_rule ? fkey : 0u;
+        const gptr<uint8_t> halo = Glob(d.halo) + (hside * d.n_blocks + bl.lb) * d.halo_stride;
+        const gptr<uint64_t> E = Glob(d.cpar);
+        const uint32_t c_max = d.n_blocks * slots, c0 = slots * bl.lb;
+        const uint64_t e_first = threadIdx.x < nslot ? CachedLoad64(&E[c0 + threadIdx.x]) : 0ull;
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k)
+            hs[k] = halo[hoff[k]];
         for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
             fb[i] = i < qmax / 32u ? RuleWord(d, bl, i, random_rule, key) : 0u;
         __syncthreads();
         // each slot's canonical root, one slot per thread; its bit starts clear (edge-reaching clusters have
         // their own bits) and is set where the rule flips the root
-        const gptr<uint64_t> E = Glob(d.cpar);
-        const uint32_t c_max = d.n_blocks * slots;
-        ForBlockSlots(d, bl, threadIdx.x, kThreads, [&](const uint32_t slot, const uint32_t c) {
-            const uint32_t root = SlotRootKey(E, c, c_max, ok);
+        for (uint32_t i = threadIdx.x; i < nslot; i += kThreads)
+        {
+            const uint32_t root = SlotRootKey(E, c0 + i, i == threadIdx.x ? e_first : CachedLoad64(&E[c0 + i]), c_max, ok);
             if (FlipBit(d, root, random_rule, key))
-                (void)__hip_atomic_fetch_or(&fb[(qcap + slot) >> 5], 1u << ((qcap + slot) & 31u), __ATOMIC_RELAXED,
+                (void)__hip_atomic_fetch_or(&fb[(qcap + i) >> 5], 1u << ((qcap + i) & 31u), __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-        });
+        }
         __syncthreads();
 
         const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * d.fbw;
@@ -1297,10 +1335,7 @@ namespace swmi
         {
             const uint32_t t = threadIdx.x + k * kThreads;
             if (t < rows + 64u)
-            {
-                const uint32_t v = (hs[k] ^ FlipOfLabel(hl[k], fb, qmax, qcap, slots, ok)) & 1u;
-                halo[t < rows ? t : d.ecol_stride + (t - rows)] = static_cast<uint8_t>(v);
-            }
+                halo[hoff[k]] = static_cast<uint8_t>((hs[k] ^ FlipOfLabel(hl[k], fb, qmax, qcap, slots, ok)) & 1u);
         }
         if (!ok)
             (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1440,7 +1475,7 @@ namespace swmi
         // phase 1: canonical root of every slot in use
         const gptr<uint64_t> E = Glob(d.cpar);
         const uint32_t c_max = d.n_blocks * slots;
-        ForBlockSlots(d, bl, threadIdx.x, 256u, [&](const uint32_t slot, const uint32_t c) { R[slot] = SlotRootKey(E, c, c_max, ok); });
+        ForBlockSlots(d, bl, threadIdx.x, 256u, [&](const uint32_t slot, const uint32_t c) { R[slot] = SlotRootKey(E, c, CachedLoad64(&E[c]), c_max, ok); });
         __syncthreads();
 
         // phase 2: every site's label, four sites per lane
