@@ -163,6 +163,7 @@ namespace swmi
         g.halo_stride = g.ecol_stride + 64u;
         g.qcap = kWavesPerBlock * g.twr * 64u;
         g.fbw = (g.qcap + kWavesPerBlock * g.edge_cap + 31u) / 32u;
+        g.fsw = (kWavesPerBlock * g.edge_cap + 31u) / 32u;
         return g;
     }
 
@@ -186,6 +187,7 @@ namespace swmi
         d.halo_stride = g.halo_stride;
         d.qcap = g.qcap;
         d.fbw = g.fbw;
+        d.fsw = g.fsw;
         return d;
     }
 
@@ -196,7 +198,7 @@ namespace swmi
         return n * (1 + 4 + 2) + groups * (3 * 8 + 4) + static_cast<uint64_t>(kBbitWords) * 8 * g.n_blocks +
                4ull * (kLatStateWords + 4ull * c.max_samples) + 4ull * 2 * kWavesPerBlock * g.n_blocks +
                static_cast<uint64_t>(kWavesPerBlock) * g.n_blocks * (8ull * g.edge_cap + 4) + 8ull * g.ecol_stride * g.n_blocks +
-               4ull * 2 * 64 * g.n_blocks + 4ull * g.fbw * g.n_blocks + 2ull * g.halo_stride * g.n_blocks;
+               4ull * 2 * 64 * g.n_blocks + 4ull * g.fsw * g.n_blocks + 2ull * g.halo_stride * g.n_blocks;
     }
 
     bool Lattice::SameShape(const swmi_lattice_config& c) const
@@ -233,7 +235,7 @@ namespace swmi
         edge_count.allocate(static_cast<size_t>(kWavesPerBlock) * g.n_blocks);
         ecol.allocate(2ull * g.ecol_stride * g.n_blocks);
         erow.allocate(2ull * 64 * g.n_blocks);
-        fbits.allocate(static_cast<size_t>(g.fbw) * g.n_blocks);
+        fbits.allocate(static_cast<size_t>(g.fsw) * g.n_blocks);
         halo.allocate(2ull * g.halo_stride * g.n_blocks);
     }
 

@@ -110,7 +110,7 @@ namespace swmi
     struct Geometry
     {
         uint32_t pitch = 0, tw = 0, rpw = 0, twr = 0, tiles_x = 0, blocks_y = 0, nwr_y = 0, n_blocks = 0, edge_cap = 0,
-                 ecol_stride = 0, halo_stride = 0, elast = 0, qcap = 0, fbw = 0;
+                 ecol_stride = 0, halo_stride = 0, elast = 0, qcap = 0, fbw = 0, fsw = 0;
         uint64_t vlast = 0;
     };
     // Row pitch of an n0 x n1 lattice: n0 when the wave64 tiling takes it as it is, else the next width it

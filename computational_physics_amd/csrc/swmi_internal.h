@@ -25,8 +25,9 @@
 //   ecol     uint32[2*ecol_stride*blocks] (label << 16) | root index q of each block's first (side 0) and
 //                            last (side 1) column, one entry per lattice row of the block, for the merge
 //   erow     uint32[2*64*blocks] the same for the block's first (side 0) and last (side 1) wave row
-//   fbits    uint32[fbw*blocks] per block: the flip bit of every cluster by its label's index (root index q,
-//                            or qcap + slot), written by k_resolve, read by the next sweep's k_bond_label
+//   fbits    uint32[fsw*blocks] per block: the flip bit of every edge-reaching cluster by its slot, written by
+//                            k_resolve, read by the next sweep's k_bond_label (the bits of the clusters inside a
+//                            block follow from the flip rule at their roots and are not stored)
 //   parent   uint32[n]       inspection only: canonical (minimum) site index of every site (k_write_labels)
 //   halo     uint8[2*(ecol_stride+64)*blocks] two sides (by sweep parity) of, per block, the spins of its
 //                            first column (one per lattice row) and of its first wave row: k_bond_label
@@ -85,6 +86,7 @@ namespace swmi
         uint32_t edge_cap;       // slots of one tile: min(tile sites, block boundary sites)
         uint32_t qcap;           // block-local indices of a block: kWavesPerBlock * twr * 64
         uint32_t fbw;            // flip-bit words per block: (qcap + 4 * edge_cap) / 32, rounded up
+        uint32_t fsw;            // slot flip-bit words of a block: 4 * edge_cap / 32, rounded up
         uint32_t n0, n1;         // row pitch (padded width) and height
         uint32_t n0v;            // lattice width: columns [n0v, n0) are padding
         uint32_t elast;          // lane-in-tile (lx) of a row's last site in the last tile column
@@ -100,7 +102,7 @@ namespace swmi
         uint32_t n_blocks;       // tiles_x * blocks_y
         uint32_t flip_rule;      // SWMI_FLIP_PARITY (0) or SWMI_FLIP_RANDOM (1)
         uint32_t seed;           // the lattice's lcg32 seed (keys the random flip rule)
-        uint32_t* fbits;         // fbw words per block
+        uint32_t* fbits;         // fsw words per block
         uint8_t* halo;           // two sides of n_blocks * halo_stride bytes: first column, then first wave row
         uint32_t halo_stride;    // ecol_stride + 64
     };

@@ -146,9 +146,9 @@ namespace swmi
     // slot, where a root's key is its site index and a non-root holds its parent's key. Links are made with
     // agent-scope 64-bit atomicMin (the 8 XCD L2s are not coherent with each other for plain stores), which
     // orders by key first: a parent's key never exceeds its child's, so a tree's root is the minimum site of
-    // its component. Chases read through the caches (workgroup-scope relaxed loads: never hoisted by the
-    // compiler, allowed to hit L1/L2); a stale value is an earlier parent, still an ancestor, and every link is
-    // decided by an atomicMin that returns the true value, so staleness only costs retries. ----
+    // its component. Chase loads are relaxed atomic loads (never hoisted by the compiler); a stale value is an
+    // earlier parent, still an ancestor, and every link is decided by an atomicMin that returns the true value,
+    // so staleness only costs retries and longer paths. ----
     __device__ __forceinline__ uint64_t SlotEntry(const uint32_t key, const uint32_t slot)
     {
         return (static_cast<uint64_t>(key) << 32) | slot;
@@ -157,6 +157,16 @@ namespace swmi
     __device__ __forceinline__ uint64_t CachedLoad64(gptr<uint64_t> p)
     {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+
+    // Loads of the merge's chases. FRESH reads the entry as it is now, past the CU's cache (agent scope);
+    // otherwise the load may hit a line the CU cached earlier in the kernel. Fresh loads cost memory traffic
+    // and save retries: they pay where the merge is latency-bound, i.e. where all its blocks are resident at
+    // once (LaunchMerge chooses).
+    template <bool FRESH>
+    __device__ __forceinline__ uint64_t ChaseLoad64(gptr<uint64_t> p)
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, FRESH ? __HIP_MEMORY_SCOPE_AGENT : __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
     __device__ __forceinline__ uint64_t SlotLinkMin(gptr<uint64_t> p, const uint64_t v)
@@ -172,6 +182,7 @@ namespace swmi
     // root is pointed at the root (the atomic returns nothing, so the wave does not wait for it). A slot out
     // of range or a parent with a larger key can only come from a broken invariant, which is reported through
     // the lattice's error flag instead of being followed.
+    template <bool FRESH>
     __device__ void SlotUnion(gptr<uint64_t> E, uint32_t a, uint32_t ka, uint32_t b, uint32_t kb, const uint32_t nslots,
                               gptr<uint32_t> err)
     {
@@ -196,7 +207,7 @@ namespace swmi
             if (a >= nslots || b >= nslots)
                 break;
             const uint32_t a0 = a, b0 = b;
-            uint64_t ea = CachedLoad64(&E[a]), eb = CachedLoad64(&E[b]);
+            uint64_t ea = ChaseLoad64<FRESH>(&E[a]), eb = ChaseLoad64<FRESH>(&E[b]);
             uint32_t ha = 0, hb = 0;
             bool bad = false;
             while (static_cast<uint32_t>(ea) != a || static_cast<uint32_t>(eb) != b)
@@ -212,14 +223,14 @@ namespace swmi
                 {
                     a = pa;
                     ka = qa;
-                    ea = CachedLoad64(&E[a]);
+                    ea = ChaseLoad64<FRESH>(&E[a]);
                     ++ha;
                 }
                 if (pb != b)
                 {
                     b = pb;
                     kb = qb;
-                    eb = CachedLoad64(&E[b]);
+                    eb = ChaseLoad64<FRESH>(&E[b]);
                     ++hb;
                 }
             }
@@ -315,6 +326,36 @@ namespace swmi
     __device__ __forceinline__ uint32_t SiteOf(const uint32_t base, const uint32_t stride, const uint32_t q)
     {
         return base + (q >> 6) * stride + (q & 63u);
+    }
+
+    // ---- the flip rule (reference FlipClusters_Kernel, swendsen_wang_amd_gpu.cpp:445-467) ----
+    // Row-major site index of a padded site index p (d.n0 is the row pitch, d.n0v the lattice width).
+    __device__ __forceinline__ uint32_t RealSite(const LatDesc& d, const uint32_t p)
+    {
+        return d.n0v == d.n0 ? p : (p / d.n0) * d.n0v + p % d.n0;
+    }
+
+    // The flip rule for the cluster whose canonical root is the padded site `root`. The pitch and the width
+    // are even, so a padded index and its row-major index have the same parity.
+    __device__ __forceinline__ uint32_t FlipBit(const LatDesc& d, const uint32_t root, const bool random_rule, const uint32_t key)
+    {
+        return random_rule ? (Fmix32(RealSite(d, root) ^ key) >> 31) : (root & 1u);
+    }
+
+    // A block's flip bitmap has one bit per label index: bit q (q < qcap) for the cluster rooted at block-local
+    // index q that stays inside the block, the rule applied to the root's own site; bit qcap + s for the
+    // edge-reaching cluster whose root holds slot s, the rule applied to its canonical root (set by the
+    // resolve). Word w of the rule part: bit j for q = 32w + j (block bases and row strides are even, so under
+    // the parity rule q's parity is its site's parity).
+    __device__ __forceinline__ uint32_t RuleWord(const LatDesc& d, const Block& bl, const uint32_t w, const bool random_rule,
+                                                 const uint32_t key)
+    {
+        if (!random_rule)
+            return 0xAAAAAAAAu;
+        uint32_t v = 0;
+        for (uint32_t j = 0; j < 32u; ++j)
+            v |= FlipBit(d, SiteOf(bl.base, bl.stride, 32u * w + j), true, key) << j;
+        return v;
     }
 
     // =============================================================================================
@@ -452,7 +493,7 @@ namespace swmi
         const Rsrc rs_spins = MakeRsrc(d.spins + bl.base, span);
         const Rsrc rs_rng = MakeRsrc(d.rng + bl.base, span * 4u);
         const Rsrc rs_local = MakeRsrc(d.local + bl.base, span * 2u);
-        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * d.fbw;
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * d.fsw;
         const uint32_t nwaves = (bl.nwr + twr - 1u) / twr;
         const bool last_wave = w + 1u == nwaves;
         const uint32_t sx_r = bl.sx + 1u == d.tiles_x ? 0u : bl.sx + 1u;   // block column to the right
@@ -463,11 +504,17 @@ namespace swmi
 
         // What needs only the descriptor and must be in place before the barrier is loaded while the lattice's
         // counters are on their way, so that the two round trips overlap: the block's flip bits (they matter
-        // only under a pending flip) and every candidate for the spin below the tile (which one is used depends
-        // on the counters).
+        // only under a pending flip: the edge-reaching clusters' bits from k_resolve, before them in FB the rule
+        // words of the clusters inside the block, constant under the parity rule) and every candidate for the
+        // spin below the tile (which one is used depends on the counters).
         if (!injected)
-            for (uint32_t i = threadIdx.x; i < d.fbw; i += 256u)
-                FB[i] = fbits_g[i];
+        {
+            for (uint32_t i = threadIdx.x; i < d.fsw; i += 256u)
+                FB[d.qcap / 32u + i] = fbits_g[i];
+            if (d.flip_rule == 0u)
+                for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)   // bl.nwr * 64 root indices, 32 per word
+                    FB[i] = RuleWord(d, bl, i, false, 0u);
+        }
         const uint32_t y_last = (bl.wyb + r_begin + (nrow > 0 ? nrow : 1u) - 1u) * rpw + lrow;
         const uint32_t below = (r_begin + nrow) * bl.stride + lx;   // the site below the tile, inside the block
         const uint32_t below_halo = (sy_b * d.tiles_x + bl.sx) * d.halo_stride + d.ecol_stride + lx;
@@ -501,6 +548,9 @@ namespace swmi
         // as it is loaded, from its label and the block's flip bits; the flipped spins are stored as the
         // rows are processed. That flip belongs to the sweep that started at iteration it0 - 2.
         const bool pending = !injected && stp[kStApplied] != it0;
+        if (pending && d.flip_rule != 0u)   // the random rule's words, with the key of the pending flip's sweep
+            for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
+                FB[i] = RuleWord(d, bl, i, true, FlipKey(d.seed, it0 - 2u));
         const uint32_t it1 = it0 + 1u;
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
@@ -1069,6 +1119,7 @@ namespace swmi
     // Bit `lane` of `bits`: a bond from a site of this block to a site of block lb_b (first site base_b), whose
     // edge entries (ecol / erow: (label << 16) | root index q) are va and vb. Both sites belong to clusters
     // that reach their block's edge, so their labels name slots.
+    template <bool FRESH>
     __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t va,
                                               const uint32_t vb, const uint32_t lb_b, const uint32_t base_b, gptr<uint32_t> err,
                                               uint64_t* H)
@@ -1120,9 +1171,10 @@ namespace swmi
             }
         }
         if (leader)
-            SlotUnion(Glob(d.cpar), ca, ka, cb, kb, d.n_blocks * slots, err);
+            SlotUnion<FRESH>(Glob(d.cpar), ca, ka, cb, kb, d.n_blocks * slots, err);
     }
 
+    template <bool FRESH>
     __global__ __launch_bounds__(128) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                    const uint32_t total_blocks)
     {
@@ -1167,7 +1219,7 @@ namespace swmi
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
                 const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
                 if (bits && act != 0u)
-                    MergeEdge(d, bl, bits, va, vb, lb_r, base_r, err, HS[0]);
+                    MergeEdge<FRESH>(d, bl, bits, va, vb, lb_r, base_r, err, HS[0]);
             }
         }
         else
@@ -1183,7 +1235,7 @@ namespace swmi
             Issued(va);
             Issued(vb);
             if (down_bits && act != 0u)
-                MergeEdge(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1]);
+                MergeEdge<FRESH>(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1]);
         }
     }
 
@@ -1192,35 +1244,6 @@ namespace swmi
     // swendsen_wang_amd_gpu.cpp:403-467), split into the decision (k_resolve, end of every sweep) and its
     // application to the spin array (the next sweep's k_bond_label, or k_flip when something reads spins)
     // =============================================================================================
-    // Row-major site index of a padded site index p (d.n0 is the row pitch, d.n0v the lattice width).
-    __device__ __forceinline__ uint32_t RealSite(const LatDesc& d, const uint32_t p)
-    {
-        return d.n0v == d.n0 ? p : (p / d.n0) * d.n0v + p % d.n0;
-    }
-
-    // The flip rule for the cluster whose canonical root is the padded site `root`. The pitch and the width
-    // are even, so a padded index and its row-major index have the same parity.
-    __device__ __forceinline__ uint32_t FlipBit(const LatDesc& d, const uint32_t root, const bool random_rule, const uint32_t key)
-    {
-        return random_rule ? (Fmix32(RealSite(d, root) ^ key) >> 31) : (root & 1u);
-    }
-
-    // A block's flip bitmap has one bit per label index: bit q (q < qcap) for the cluster rooted at block-local
-    // index q that stays inside the block, the rule applied to the root's own site; bit qcap + s for the
-    // edge-reaching cluster whose root holds slot s, the rule applied to its canonical root (set by the
-    // resolve). Word w of the rule part: bit j for q = 32w + j (block bases and row strides are even, so under
-    // the parity rule q's parity is its site's parity).
-    __device__ __forceinline__ uint32_t RuleWord(const LatDesc& d, const Block& bl, const uint32_t w, const bool random_rule,
-                                                 const uint32_t key)
-    {
-        if (!random_rule)
-            return 0xAAAAAAAAu;
-        uint32_t v = 0;
-        for (uint32_t j = 0; j < 32u; ++j)
-            v |= FlipBit(d, SiteOf(bl.base, bl.stride, 32u * w + j), true, key) << j;
-        return v;
-    }
-
     // Bitmap index of a label: the root index, or qcap + slot for an edge-reaching cluster. An index outside
     // the block's range (a broken invariant) clears ok and reads bit 0.
     __device__ __forceinline__ uint32_t LabelBit(const uint32_t lab, const uint32_t qmax, const uint32_t qcap, const uint32_t slots,
@@ -1327,9 +1350,9 @@ namespace swmi
         }
         __syncthreads();
 
-        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * d.fbw;
-        for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
-            fbits_g[i] = fb[i];
+        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * d.fsw;   // the edge-reaching clusters' bits
+        for (uint32_t i = threadIdx.x; i < d.fsw; i += kThreads)
+            fbits_g[i] = fb[qcap / 32u + i];
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k)
         {
@@ -1391,9 +1414,15 @@ namespace swmi
             }
         };
         load(0u);
-        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * d.fbw;
-        for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
-            fb[i] = fbits_g[i];
+        // the edge-reaching clusters' bits from k_resolve, and the rule words of the clusters inside the block
+        // (the pending flip belongs to the sweep that started at iteration `applied`)
+        const gptr<const uint32_t> fbits_g = Glob(static_cast<const uint32_t*>(d.fbits)) + bl.lb * d.fsw;
+        for (uint32_t i = threadIdx.x; i < d.fsw; i += kThreads)
+            fb[qcap / 32u + i] = fbits_g[i];
+        const bool random_rule = d.flip_rule != 0u;
+        const uint32_t key = random_rule ? FlipKey(d.seed, stp[kStApplied]) : 0u;
+        for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += kThreads)
+            fb[i] = RuleWord(d, bl, i, random_rule, key);
         __syncthreads();
 
         for (uint32_t r0 = 0;;)
@@ -1830,9 +1859,15 @@ namespace swmi
             LaunchBondLabelT<false, false>(d_desc, d_block_lat, total_blocks, max_twr, pad, stream);
     }
 
+    // Two-wave merge workgroups resident at once on MI355X: 256 CUs x 16 (8 waves per SIMD).
+    constexpr uint32_t kOneRoundMergeBlocks = 4096;
+
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
     {
-        k_merge<<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+        if (total_blocks <= kOneRoundMergeBlocks)
+            k_merge<true><<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+        else
+            k_merge<false><<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
     }
 
     void LaunchResolve(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
