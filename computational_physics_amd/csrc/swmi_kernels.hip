@@ -1286,7 +1286,7 @@ namespace swmi
     {
         constexpr uint32_t kThreads = 128;
         constexpr uint32_t kMaxSites = kWavesPerBlock * 64u * 64u;
-        __shared__ uint32_t fb[2u * kMaxSites / 32u];
+        __shared__ uint32_t sw[kMaxSites / 32u];   // flip bits of the block's slots (at most one slot per site)
         const uint32_t b = RemapBlock(total_blocks);
         if (b >= total_blocks)
             return;
@@ -1336,29 +1336,59 @@ namespace swmi
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k)
             hs[k] = halo[hoff[k]];
-        for (uint32_t i = threadIdx.x; i < d.fbw; i += kThreads)
-            fb[i] = i < qmax / 32u ? RuleWord(d, bl, i, random_rule, key) : 0u;
-        __syncthreads();
-        // each slot's canonical root, one slot per thread; its bit starts clear (edge-reaching clusters have
-        // their own bits) and is set where the rule flips the root
-        for (uint32_t i = threadIdx.x; i < nslot; i += kThreads)
+        // each slot's canonical root, one slot per lane; a wave assembles the flip bits of 64 slots per pass
+        // with a ballot and writes them as whole words, into LDS for the halo flips and into the block's fbits
+        // (words past the last slot are left as they were: no label names a slot past the count)
+        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * d.fsw;
+        const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+        for (uint32_t s0 = 0; s0 < nslot; s0 += kThreads)
         {
-            const uint32_t root = SlotRootKey(E, c0 + i, i == threadIdx.x ? e_first : CachedLoad64(&E[c0 + i]), c_max, ok);
-            if (FlipBit(d, root, random_rule, key))
-                (void)__hip_atomic_fetch_or(&fb[(qcap + i) >> 5], 1u << ((qcap + i) & 31u), __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t i = s0 + threadIdx.x;
+            uint32_t bit = 0;
+            if (i < nslot)
+                bit = FlipBit(d, SlotRootKey(E, c0 + i, i == threadIdx.x ? e_first : CachedLoad64(&E[c0 + i]), c_max, ok),
+                              random_rule, key);
+            const uint64_t m = __ballot(bit != 0u);
+            const uint32_t wd = (s0 + w * 64u) >> 5;
+            if (lane == 0u && wd < d.fsw)
+            {
+                sw[wd] = static_cast<uint32_t>(m);
+                fbits_g[wd] = static_cast<uint32_t>(m);
+            }
+            if (lane == 32u && wd + 1u < d.fsw)
+            {
+                sw[wd + 1u] = static_cast<uint32_t>(m >> 32);
+                fbits_g[wd + 1u] = static_cast<uint32_t>(m >> 32);
+            }
         }
         __syncthreads();
 
-        const gptr<uint32_t> fbits_g = Glob(d.fbits) + bl.lb * d.fsw;   // the edge-reaching clusters' bits
-        for (uint32_t i = threadIdx.x; i < d.fsw; i += kThreads)
-            fbits_g[i] = fb[qcap / 32u + i];
+        // the flip of a halo site: its slot's bit, or for a cluster inside the block (a padding lane of the
+        // first wave row) the rule at its root; a label out of range clears ok
+        const auto halo_flip = [&](const uint32_t lab) -> uint32_t {
+            const uint32_t i = lab & 0x7FFFu;
+            if (lab & kEdgeFlag)
+            {
+                if (i < qcap || i - qcap >= nslot)
+                {
+                    ok = false;
+                    return 0u;
+                }
+                return (sw[(i - qcap) >> 5] >> ((i - qcap) & 31u)) & 1u;
+            }
+            if (i >= qmax)
+            {
+                ok = false;
+                return 0u;
+            }
+            return FlipBit(d, SiteOf(bl.base, bl.stride, i), random_rule, key);
+        };
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k)
         {
             const uint32_t t = threadIdx.x + k * kThreads;
             if (t < rows + 64u)
-                halo[hoff[k]] = static_cast<uint8_t>((hs[k] ^ FlipOfLabel(hl[k], fb, qmax, qcap, slots, ok)) & 1u);
+                halo[hoff[k]] = static_cast<uint8_t>((hs[k] ^ halo_flip(hl[k])) & 1u);
         }
         if (!ok)
             (void)__hip_atomic_fetch_or(stp + kStErr, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
