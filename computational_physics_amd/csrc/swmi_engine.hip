@@ -27,11 +27,14 @@ namespace swmi
         std::mutex g_tune_mutex;
         std::map<std::pair<uint32_t, uint32_t>, uint32_t> g_tuned;   // (n0, n1) -> tile rows
 
-        // Sweeps per captured graph of plain sweeps; a sweep loop of fewer sweeps runs eagerly.
+        // Sweeps per captured graph of plain sweeps; a sweep loop of fewer sweeps runs eagerly. Loops of at least
+        // two long units replay long graphs (each replay boundary costs a few microseconds of idle GPU), the
+        // rest of the loop short ones.
         constexpr uint32_t kGraphSweeps = 16;
+        constexpr uint32_t kLongGraphSweeps = 128;
         // Captured measurement graphs hold groups of (n_sep sweeps + sample) up to about this many sweeps.
         constexpr uint32_t kGraphGroupSweeps = 64;
-        constexpr size_t kMaxGraphs = 4;
+        constexpr size_t kMaxGraphs = 6;
     }
 
     void SetLastError(const std::string& m) { g_error = m; }
@@ -482,7 +485,7 @@ void swmi_engine::Run(uint32_t n, uint32_t sep, int32_t sample_mode)
     }
     // one replay unit: `unit` sweeps, or `groups` groups of (sep sweeps + sample)
     const uint32_t groups = sampled ? std::max<uint32_t>(1u, kGraphGroupSweeps / sep) : 1u;
-    const uint32_t unit = sampled ? groups * sep : kGraphSweeps;
+    const uint32_t unit = sampled ? groups * sep : (n >= 2u * kLongGraphSweeps ? kLongGraphSweeps : kGraphSweeps);
     const uint32_t total_groups = sampled ? (n + sep - 1) / sep : 0u;   // ising_2d.cpp:132: i += N_Sep while i < n
     const uint32_t replays = sampled ? total_groups / groups : n / unit;
     const auto enqueue_unit = [&](uint32_t g_count) {
@@ -539,7 +542,7 @@ void swmi_engine::Run(uint32_t n, uint32_t sep, int32_t sample_mode)
         if (sampled)
             enqueue_unit(total_groups - replays * groups);
         else
-            enqueue_unit(n - replays * unit);
+            Run(n - replays * unit, 0, -1);   // short graphs or eager sweeps
         return;
     }
     enqueue_unit(sampled ? total_groups : n);

@@ -51,6 +51,23 @@ def test_graph_replays_equal_eager(swmi):
             assert g.error_flags(i) == 0
 
 
+def test_long_graph_replays_equal_eager(swmi):
+    """A long sweep loop replays 128-sweep graphs and finishes with 16-sweep graphs and eager sweeps: 300 sweeps
+    are two long replays, two short ones and 12 eager sweeps, twice in a row on the same graphs."""
+    cfgs = [_cfg(swmi, n0, n1, T, seed=i + 31, spin_seed=i + 32, tile_rows=tr) for i, (n0, n1, T, tr) in enumerate(MIXED[:4])]
+    print("long graph vs eager", flush=True)
+    with swmi.Engine(cfgs) as g, swmi.Engine(cfgs) as e:
+        e.set_graphs(False)
+        for rep in range(2):
+            g.sweep(300)
+            e.sweep(300)
+            for i in range(len(cfgs)):
+                np.testing.assert_array_equal(g.spins(i), e.spins(i), err_msg=f"sweep loop {rep} lattice {i}")
+        for i in range(len(cfgs)):
+            assert g.rng_state(i)["iteration"] == e.rng_state(i)["iteration"] == 2 * 600
+            assert g.error_flags(i) == 0
+
+
 def test_mixed_batch_equals_alone_and_oracle(swmi, oracle):
     """Lattices of every kernel class (narrow and 64-wide tiles, tile heights 1..64) in one engine."""
     cfgs = [_cfg(swmi, n0, n1, T, seed=10 + i, spin_seed=20 + i, tile_rows=tr) for i, (n0, n1, T, tr) in enumerate(MIXED)]
