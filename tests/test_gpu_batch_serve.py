@@ -68,6 +68,33 @@ def test_long_graph_replays_equal_eager(swmi):
             assert g.error_flags(i) == 0
 
 
+def test_graph_cache_eviction_equals_eager(swmi):
+    """More graph shapes than the engine keeps (plain 16- and 128-sweep units, and sampled runs with six
+    different n_sep): the cache is emptied while earlier replays may still be queued, and every result still
+    equals the eager engine's, spins, moments and series alike."""
+    cfgs = [_cfg(swmi, n0, n1, T, seed=i + 41, spin_seed=i + 42, tile_rows=tr, max_samples=512)
+            for i, (n0, n1, T, tr) in enumerate(MIXED[:3])]
+    print("graph cache eviction vs eager", flush=True)
+    with swmi.Engine(cfgs) as g, swmi.Engine(cfgs) as e:
+        e.set_graphs(False)
+        steps = [("sweep", 32), ("sweep", 256)] + [("run", sep) for sep in (2, 3, 4, 5, 6, 7)]
+        for kind, k in steps:
+            for eng in (g, e):
+                if kind == "sweep":
+                    eng.sweep(k)
+                else:
+                    eng.run(128, k)
+            for i in range(len(cfgs)):
+                np.testing.assert_array_equal(g.spins(i), e.spins(i), err_msg=f"{kind} {k} lattice {i}")
+                np.testing.assert_array_equal(g.moments(i), e.moments(i), err_msg=f"{kind} {k} lattice {i}")
+                ge, gm = g.series(i)
+                ee, em = e.series(i)
+                np.testing.assert_array_equal(ge, ee)
+                np.testing.assert_array_equal(gm, em)
+        for i in range(len(cfgs)):
+            assert g.error_flags(i) == 0
+
+
 def test_mixed_batch_equals_alone_and_oracle(swmi, oracle):
     """Lattices of every kernel class (narrow and 64-wide tiles, tile heights 1..64) in one engine."""
     cfgs = [_cfg(swmi, n0, n1, T, seed=10 + i, spin_seed=20 + i, tile_rows=tr) for i, (n0, n1, T, tr) in enumerate(MIXED)]
@@ -299,3 +326,33 @@ def test_server_rejects_invalid_jobs(swmi, bad):
         with pytest.raises(swmi.SwmiError):
             srv.submit(swmi.Job(**base))
         assert srv.stats()["submitted"] == 0
+
+
+def test_bench_batch_equals_lone_runs_and_oracle(swmi, oracle):
+    """The bench's batched workload itself (bench.batch_workload: 256 lattices of 32^2 .. 1024^2, default_rng(0)),
+    in one engine at one tile height: after 6 sweeps every lattice equals its lone run (each at the height tuned
+    for its extent), and every lattice up to 128^2 equals the oracle's restatement of the per-site sweep."""
+    import bench
+    cfgs = bench.batch_workload(swmi)
+    assert len(cfgs) == 256
+    n = 6
+    for c in cfgs:
+        c.tile_rows = 16
+    print(f"bench batch: {len(cfgs)} lattices, {sum(c.n0 * c.n1 for c in cfgs)} sites", flush=True)
+    with swmi.Engine(cfgs) as batch:
+        batch.sweep(n)
+        got = [batch.spins(i) for i in range(len(cfgs))]
+        assert all(batch.error_flags(i) == 0 for i in range(len(cfgs)))
+    small = 0
+    for i, c in enumerate(cfgs):
+        c.tile_rows = 0
+        with swmi.Engine(c) as e:
+            e.sweep(n)
+            np.testing.assert_array_equal(got[i], e.spins(), err_msg=f"lattice {i} ({c.n0}^2)")
+        if c.n0 <= 128:
+            ref = oracle.PerSite(c.n0, c.n1, c.temperature, seed=c.seed, spin_seed=c.spin_seed)
+            for _ in range(n):
+                ref.sweep()
+            np.testing.assert_array_equal(got[i], ref.spins, err_msg=f"lattice {i} ({c.n0}^2) vs oracle")
+            small += 1
+    assert small > 64
