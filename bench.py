@@ -242,13 +242,17 @@ def accuracy_leg(cpa, n: int = 4096, warmup: int = 1000, sweeps: int = 4000, sep
     return out
 
 
-def plain_configs_leg(cpa, warmup: int = 50, sweeps: int = 300) -> dict:
+def plain_configs_leg(cpa, warmup=(256, 32), sweeps: int = 300) -> dict:
     """The other plain-path configurations: 1024^2 at T_c with the default and the autotuned tile height,
     4096^2 and 8192^2 at T in {2.0, 2.2691853, 2.6}; sweep-only rates and the 4096^2 rate of the reference
-    protocol (observables every 10 sweeps inside the timed region)."""
+    protocol (observables every 10 sweeps inside the timed region). The untimed warmup loops capture the
+    graphs the timed loop replays (128- and 16-sweep units; with observables, the sample-group graph)."""
     def rate(cfg, n_sweeps, sep=0):
         with cpa.Engine(cfg) as e:
-            e.sweep(warmup)
+            for w in warmup:
+                e.sweep(w)
+            if sep:
+                e.run(128, sep)
             e.synchronize()
             t0 = time.perf_counter()
             if sep:
