@@ -167,6 +167,9 @@ namespace swmi
         g.qcap = kWavesPerBlock * g.twr * 64u;
         g.fbw = (g.qcap + kWavesPerBlock * g.edge_cap + 31u) / 32u;
         g.fsw = (kWavesPerBlock * g.edge_cap + 31u) / 32u;
+        // the bond pass loads a block's slot flip bits one word per thread of its 256
+        if (g.fsw > 256u)
+            throw SwmiError(SWMI_ERR_INVALID, "block slot bitmap of " + std::to_string(g.fsw) + " words exceeds 256");
         return g;
     }
 

@@ -136,6 +136,14 @@ namespace swmi
         asm volatile("" ::"v"(v));
     }
 
+    // An empty asm statement that needs the uniform value v in scalar registers here: a descriptor field read
+    // this way is fetched at this point, with the others read beside it, not at its first use.
+    template <typename T>
+    __device__ __forceinline__ void HeldScalar(const T v)
+    {
+        asm volatile("" ::"s"(v));
+    }
+
     __device__ __forceinline__ void LdsMin(uint32_t* p, const uint32_t v)
     {
         (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -502,38 +510,56 @@ namespace swmi
         const gptr<const uint8_t> halo_s0 = Glob(static_cast<const uint8_t*>(d.halo));
         const gptr<const uint8_t> halo_s1 = halo_s0 + d.n_blocks * d.halo_stride;
 
-        // What needs only the descriptor and must be in place before the barrier is loaded while the lattice's
-        // counters are on their way, so that the two round trips overlap: the block's flip bits (they matter
-        // only under a pending flip: the edge-reaching clusters' bits from k_resolve, before them in FB the rule
-        // words of the clusters inside the block, constant under the parity rule) and every candidate for the
-        // spin below the tile (which one is used depends on the counters).
+        // Everything the block reads that needs only the descriptor is loaded together, before any of it is
+        // used, so that the round trips overlap: the lattice's counters (read before this kernel stores any
+        // state), the block's flip bits (they matter only under a pending flip: the edge-reaching clusters'
+        // bits from k_resolve; before them in FB the rule words of the clusters inside the block, constant
+        // under the parity rule) and every candidate for the spin below the tile (which one is used depends on
+        // the counters). The descriptor fields these loads need are fetched first, in one batch, and the loads
+        // carry no condition (in-range addresses, values unused where they do not apply), so that the compiler
+        // neither fetches a field at its first use nor waits for every load at the end of a branch.
+        HeldScalar(d.state);
+        HeldScalar(d.fbits);
+        HeldScalar(d.halo);
+        HeldScalar(d.ecol_stride);
+        HeldScalar(d.halo_stride);
+        HeldScalar(d.qcap);
+        HeldScalar(d.fsw);
+        HeldScalar(d.n_blocks);
+        const uint32_t it0 = injected ? 0u : stp[kStIteration];
+        const uint32_t it_end = injected ? 0u : stp[kStItEnd];
+        const uint32_t applied = injected ? 0u : stp[kStApplied];
+        const uint32_t y_last = (bl.wyb + r_begin + (nrow > 0 ? nrow : 1u) - 1u) * rpw + lrow;
+        const uint32_t y_below = nrow > 0 ? ((y_last + 1u == n1) ? 0u : y_last + 1u) : 0u;
+        const uint32_t below = (r_begin + nrow) * bl.stride + lx;   // the site below the tile, inside the block
+        const uint32_t below_halo = (sy_b * d.tiles_x + bl.sx) * d.halo_stride + d.ecol_stride + lx;
+        const uint32_t hd_spin = spins[y_below * n0 + x];
+        // (the label-only pass over injected bonds has no flip bits and no halos: those loads belong to the sweep,
+        // where `injected` is false at compile time)
+        uint32_t fbw = 0, hd_lab = 0, hd_sv = 0, hd_h0 = 0, hd_h1 = 0;
         if (!injected)
         {
-            for (uint32_t i = threadIdx.x; i < d.fsw; i += 256u)
-                FB[d.qcap / 32u + i] = fbits_g[i];
+            // fsw <= 256: a block has at most 2 * 64 + 2 * kWavesPerBlock * 64 boundary sites, 4 * edge_cap / 32
+            // words of slots (the engine checks the bound), so one word per thread
+            fbw = fbits_g[min(static_cast<uint32_t>(threadIdx.x), d.fsw - 1u)];
+            hd_lab = __builtin_amdgcn_raw_buffer_load_b16(rs_local, below * 2u, 0, 0);
+            hd_sv = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, below, 0, 0);
+            hd_h0 = halo_s0[below_halo];
+            hd_h1 = halo_s1[below_halo];
+            Issued(fbw);   // not sunk into the branch that stores it
+        }
+
+        if (!injected)
+        {
+            if (threadIdx.x < d.fsw)
+                FB[d.qcap / 32u + threadIdx.x] = fbw;
             if (d.flip_rule == 0u)
                 for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)   // bl.nwr * 64 root indices, 32 per word
                     FB[i] = RuleWord(d, bl, i, false, 0u);
         }
-        const uint32_t y_last = (bl.wyb + r_begin + (nrow > 0 ? nrow : 1u) - 1u) * rpw + lrow;
-        const uint32_t below = (r_begin + nrow) * bl.stride + lx;   // the site below the tile, inside the block
-        const uint32_t below_halo = (sy_b * d.tiles_x + bl.sx) * d.halo_stride + d.ecol_stride + lx;
-        uint32_t hd_spin = 0, hd_lab = 0, hd_sv = 0, hd_h0 = 0, hd_h1 = 0;
-        if (nrow > 0)
-        {
-            hd_spin = spins[((y_last + 1u == n1) ? 0u : y_last + 1u) * n0 + x];
-            if (!injected)
-            {
-                hd_lab = __builtin_amdgcn_raw_buffer_load_b16(rs_local, below * 2u, 0, 0);
-                hd_sv = __builtin_amdgcn_raw_buffer_load_b8(rs_spins, below, 0, 0);
-                hd_h0 = halo_s0[below_halo];
-                hd_h1 = halo_s1[below_halo];
-            }
-        }
 
         // sweep schedule: a lattice past its last sweep is left alone by the whole sweep
-        const uint32_t it0 = injected ? 0u : stp[kStIteration];
-        const bool act = injected || it0 < stp[kStItEnd];
+        const bool act = injected || it0 < it_end;
         if (bl.lb == 0 && threadIdx.x == 0)
         {
             stp[kStActive] = act ? 1u : 0u;
@@ -547,7 +573,7 @@ namespace swmi
         // The previous sweep's flip is still pending (applied != iteration): it is applied here, each site
         // as it is loaded, from its label and the block's flip bits; the flipped spins are stored as the
         // rows are processed. That flip belongs to the sweep that started at iteration it0 - 2.
-        const bool pending = !injected && stp[kStApplied] != it0;
+        const bool pending = !injected && applied != it0;
         if (pending && d.flip_rule != 0u)   // the random rule's words, with the key of the pending flip's sweep
             for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
                 FB[i] = RuleWord(d, bl, i, true, FlipKey(d.seed, it0 - 2u));
