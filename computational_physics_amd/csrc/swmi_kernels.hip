@@ -549,15 +549,6 @@ namespace swmi
             Issued(fbw);   // not sunk into the branch that stores it
         }
 
-        if (!injected)
-        {
-            if (threadIdx.x < d.fsw)
-                FB[d.qcap / 32u + threadIdx.x] = fbw;
-            if (d.flip_rule == 0u)
-                for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)   // bl.nwr * 64 root indices, 32 per word
-                    FB[i] = RuleWord(d, bl, i, false, 0u);
-        }
-
         // sweep schedule: a lattice past its last sweep is left alone by the whole sweep
         const bool act = injected || it0 < it_end;
         if (bl.lb == 0 && threadIdx.x == 0)
@@ -574,9 +565,6 @@ namespace swmi
         // as it is loaded, from its label and the block's flip bits; the flipped spins are stored as the
         // rows are processed. That flip belongs to the sweep that started at iteration it0 - 2.
         const bool pending = !injected && applied != it0;
-        if (pending && d.flip_rule != 0u)   // the random rule's words, with the key of the pending flip's sweep
-            for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
-                FB[i] = RuleWord(d, bl, i, true, FlipKey(d.seed, it0 - 2u));
         const uint32_t it1 = it0 + 1u;
         const uint32_t it2 = it1 + 1u;
         const bool rot1 = !injected && (it1 % 15u == 0u);
@@ -625,34 +613,48 @@ namespace swmi
         // spin) and flipped after it, with the block's flip bits.
         uint32_t halo_down = 0;
         const bool carry = nrow > 0 && pending && !last_wave;
-        if (nrow > 0)
-        {
-            if (!pending)
-                halo_down = hd_spin;
-            else if (carry)
-                __hip_atomic_store(dummy, (hd_lab << 8) | (hd_sv & 0xFFu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            else
-                halo_down = hside ? hd_h0 : hd_h1;
-        }
-        __syncthreads();   // AC, FB, the carried site below
-        if (carry)
-        {
-            const uint32_t v = LdsLoad(dummy);
-            const uint32_t lab = v >> 8;
-            halo_down = (v & 0xFFu) ^ flip_of(lab);
-        }
+        // The block's flip bits and rule words, the carried site below and the barrier that publishes them: run
+        // by every wave after the loads of its first chunk of rows are issued, so that this work and the
+        // barrier overlap their round trip.
+        const auto meet = [&]() {
+            if (!injected)
+            {
+                if (threadIdx.x < d.fsw)
+                    FB[d.qcap / 32u + threadIdx.x] = fbw;
+                if (d.flip_rule == 0u)
+                    for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)   // bl.nwr * 64 root indices, 32 per word
+                        FB[i] = RuleWord(d, bl, i, false, 0u);
+                else if (pending)   // the random rule's words, with the key of the pending flip's sweep
+                    for (uint32_t i = threadIdx.x; i < bl.nwr * 2u; i += 256u)
+                        FB[i] = RuleWord(d, bl, i, true, FlipKey(d.seed, it0 - 2u));
+            }
+            if (nrow > 0)
+            {
+                if (!pending)
+                    halo_down = hd_spin;
+                else if (carry)
+                    __hip_atomic_store(dummy, (hd_lab << 8) | (hd_sv & 0xFFu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                    halo_down = hside ? hd_h0 : hd_h1;
+            }
+            __syncthreads();   // AC, FB, the carried site below
+            if (carry)
+            {
+                const uint32_t v = LdsLoad(dummy);
+                const uint32_t lab = v >> 8;
+                halo_down = (v & 0xFFu) ^ flip_of(lab);
+            }
+        };
 
         uint64_t last_B = 0, last_D = 0, down_bits = 0, right_bits = 0;
         uint32_t qlen = 0;
         uint32_t Rp = 0;   // 64-wide tiles: root of this lane's site in the previous row
-        if (nrow > 0)
+        uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
+        uint32_t prev_site = 0;
+        uint32_t colv = 0;   // 64-wide tiles: lane r collects the first-column spin of row r
         {
-
-            uint64_t prev_S = 0, prev_U2 = 0, prev_B = 0, prev_Ball = 0;
-            uint32_t prev_site = 0;
             // the chunk loop exists twice: sweeps without a group rotation (13 in 15) carry no rotation code
             // (and twice more for the sweeps that apply a pending flip on load, the rule after any sweep)
-            uint32_t colv = 0;   // 64-wide tiles: lane r collects the first-column spin of row r
             const auto chunks = [&](auto rotating, auto flipping) {
             constexpr bool kRot = decltype(rotating)::value;
             constexpr bool kPend = decltype(flipping)::value;
@@ -860,8 +862,16 @@ namespace swmi
                     }
                 }
             };
-            // a tile of at most kC rows is one chunk, starting at row 0
-            for (uint32_t r0 = 0; r0 < (TWR_MAX <= kC ? 1u : nrow); r0 += kC)
+            // a tile of at most kC rows is one chunk, starting at row 0; the first chunk's loads go out before
+            // the barrier (a wave without rows loads nothing and only meets the others there)
+            Chunk c0;
+            if (nrow > 0)
+                load(0u, c0);
+            meet();
+            if (nrow == 0)
+                return;
+            process(0u, c0);
+            for (uint32_t r0 = kC; r0 < (TWR_MAX <= kC ? 0u : nrow); r0 += kC)
             {
                 Chunk c;
                 load(r0, c);
@@ -879,6 +889,9 @@ namespace swmi
                 chunks(std::false_type{}, std::true_type{});
             else
                 chunks(std::false_type{}, std::false_type{});
+        }
+        if (nrow > 0)
+        {
             if (TW64 && !injected)   // first column of the tile's rows (rpw == 1): one store per wave
                 __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(colv & 1u), rs_halo, lane < nrow ? r_begin + lane : kNoStore, 0, 0);
 
