@@ -1158,10 +1158,43 @@ namespace swmi
     // Bit `lane` of `bits`: a bond from a site of this block to a site of block lb_b (first site base_b), whose
     // edge entries (ecol / erow: (label << 16) | root index q) are va and vb. Both sites belong to clusters
     // that reach their block's edge, so their labels name slots.
+    // Per wave, the distinct slots of one edge's root pairs in LDS: S[p] = lattice-wide slot + 1 of local node p
+    // (0: free), K[p] its key; L[p] = (key of its parent << 32) | local parent (a root holds its own key and
+    // position). Inserted by open addressing over 128 positions (at most 128 slots: every insert ends).
+    __device__ __forceinline__ uint32_t LocalNode(uint32_t* S, uint32_t* K, uint64_t* L, const uint32_t c, const uint32_t k, const bool want)
+    {
+        uint32_t p = Fmix32(c * 0x85EBCA6Bu) & 127u;
+        bool open = want;
+        while (__ballot(open))
+        {
+            if (open)
+            {
+                uint32_t found = 0u;
+                if (__hip_atomic_compare_exchange_strong(&S[p], &found, c + 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP))
+                {
+                    K[p] = k;
+                    __hip_atomic_store(&L[p], SlotEntry(k, p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    open = false;
+                }
+                else if (found == c + 1u)
+                    open = false;
+                else
+                    p = (p + 1u) & 127u;
+            }
+        }
+        return p;
+    }
+
+    __device__ __forceinline__ uint64_t LocalEntry(uint64_t* L, const uint32_t p)
+    {
+        return __hip_atomic_load(&L[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+
     template <bool FRESH>
     __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t va,
                                               const uint32_t vb, const uint32_t lb_b, const uint32_t base_b, gptr<uint32_t> err,
-                                              uint64_t* H)
+                                              uint64_t* H, uint32_t* S, uint32_t* K, uint32_t* W)
     {
         const uint32_t lane = threadIdx.x & 63u;
         const bool bit = LaneBit(bits);
@@ -1209,8 +1242,92 @@ namespace swmi
                 }
             }
         }
-        if (leader)
-            SlotUnion<FRESH>(Glob(d.cpar), ca, ka, cb, kb, d.n_blocks * slots, err);
+        if (!__ballot(leader))
+            return;
+        // The pairs of one edge often share a slot (a cluster of one block meets several of its neighbour's), and
+        // lanes that link the same root race on one entry of the slot union-find. So the pairs are first united in
+        // LDS, over their distinct slots (H now holds the local entries), and each slot that is not its local
+        // component's root is then linked under that root: one lattice-wide union per lane, no two on one slot.
+        S[lane] = 0u;
+        S[64u + lane] = 0u;
+        uint32_t a = LocalNode(S, K, H, ca, ka, leader);
+        uint32_t b = LocalNode(S, K, H, cb, kb, leader);
+        bool open = leader;
+        while (__ballot(open))
+        {
+            if (open)
+            {
+                uint64_t ea = LocalEntry(H, a), eb = LocalEntry(H, b);
+                while (static_cast<uint32_t>(ea) != a)
+                {
+                    a = static_cast<uint32_t>(ea) & 127u;
+                    ea = LocalEntry(H, a);
+                }
+                while (static_cast<uint32_t>(eb) != b)
+                {
+                    b = static_cast<uint32_t>(eb) & 127u;
+                    eb = LocalEntry(H, b);
+                }
+                if (a == b)
+                    open = false;
+                else
+                {
+                    if ((ea >> 32) > (eb >> 32))   // roots hold their own keys; link the larger under the smaller
+                    {
+                        const uint32_t t = a;
+                        a = b;
+                        b = t;
+                        const uint64_t te = ea;
+                        ea = eb;
+                        eb = te;
+                    }
+                    const uint64_t old = __hip_atomic_fetch_min(&H[b], SlotEntry(static_cast<uint32_t>(ea >> 32), a),
+                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (old == eb)
+                        open = false;
+                    else   // b had been linked under old's parent (which this link may have displaced): unite with it
+                        b = static_cast<uint32_t>(old) & 127u;
+                }
+            }
+        }
+        // the local nodes that are not their component's root, listed one per lane
+        uint32_t item[2], root[2];
+        bool has[2];
+#pragma unroll
+        for (uint32_t h = 0; h < 2u; ++h)
+        {
+            const uint32_t p = lane + 64u * h;
+            uint32_t r = p;
+            has[h] = false;
+            if (S[p] != 0u)
+            {
+                uint64_t e = LocalEntry(H, r);
+                while (static_cast<uint32_t>(e) != r)
+                {
+                    r = static_cast<uint32_t>(e) & 127u;
+                    e = LocalEntry(H, r);
+                }
+                has[h] = r != p;
+            }
+            item[h] = p;
+            root[h] = r;
+        }
+        const uint64_t f0 = __ballot(has[0]), f1 = __ballot(has[1]);
+        const uint32_t n0 = static_cast<uint32_t>(__popcll(f0)), n = n0 + static_cast<uint32_t>(__popcll(f1));
+        const uint32_t i0 = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(f0 >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(f0), 0u));
+        const uint32_t i1 = n0 + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(f1 >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(f1), 0u));
+        // at most one per local link, so at most one per pair: n <= 64
+        if (has[0] && i0 < 64u)
+            W[i0] = (root[0] << 8) | item[0];
+        if (has[1] && i1 < 64u)
+            W[i1] = (root[1] << 8) | item[1];
+        if (lane < n && lane < 64u)
+        {
+            const uint32_t w = W[lane], p = w & 0xFFu, r = w >> 8;
+            SlotUnion<FRESH>(Glob(d.cpar), S[p] - 1u, K[p], S[r] - 1u, K[r], d.n_blocks * slots, err);
+        }
+        if (n > 64u && lane == 0u)
+            (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
     template <bool FRESH>
@@ -1229,7 +1346,8 @@ namespace swmi
         const uint32_t w = Uniform(threadIdx.x >> 6);
         const uint32_t tw = d.tw, rpw = d.rpw, twr = d.twr;
         const gptr<uint32_t> err = Glob(d.state) + kStErr;
-        __shared__ uint64_t HS[2][128];   // per wave: hash set of the root pairs of one edge pass
+        __shared__ uint64_t HS[2][128];   // per wave: hash set of the root pairs of one edge pass, then the local entries
+        __shared__ uint32_t SS[2][128], SK[2][128], SW[2][64];   // per wave: local slots, their keys, the union list
 
         if (w == 0)
         {
@@ -1258,7 +1376,7 @@ namespace swmi
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
                 const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
                 if (bits && act != 0u)
-                    MergeEdge<FRESH>(d, bl, bits, va, vb, lb_r, base_r, err, HS[0]);
+                    MergeEdge<FRESH>(d, bl, bits, va, vb, lb_r, base_r, err, HS[0], SS[0], SK[0], SW[0]);
             }
         }
         else
@@ -1274,7 +1392,7 @@ namespace swmi
             Issued(va);
             Issued(vb);
             if (down_bits && act != 0u)
-                MergeEdge<FRESH>(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1]);
+                MergeEdge<FRESH>(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1], SS[1], SK[1], SW[1]);
         }
     }
 
