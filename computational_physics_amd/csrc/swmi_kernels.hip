@@ -1191,7 +1191,7 @@ namespace swmi
         return __hip_atomic_load(&L[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
-    template <bool FRESH>
+    template <bool FRESH, bool LOCAL>
     __device__ __forceinline__ void MergeEdge(const LatDesc& d, const Block& bl, const uint64_t bits, const uint32_t va,
                                               const uint32_t vb, const uint32_t lb_b, const uint32_t base_b, gptr<uint32_t> err,
                                               uint64_t* H, uint32_t* S, uint32_t* K, uint32_t* W)
@@ -1241,6 +1241,12 @@ namespace swmi
                         slot = (slot + 1u) & 127u;
                 }
             }
+        }
+        if constexpr (!LOCAL)
+        {
+            if (leader)
+                SlotUnion<FRESH>(Glob(d.cpar), ca, ka, cb, kb, d.n_blocks * slots, err);
+            return;
         }
         if (!__ballot(leader))
             return;
@@ -1330,7 +1336,7 @@ namespace swmi
             (void)__hip_atomic_fetch_or(err, kErrLabelRange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
-    template <bool FRESH>
+    template <bool FRESH, bool LOCAL>
     __global__ __launch_bounds__(128) void k_merge(const LatDesc* __restrict__ descs, const uint32_t* __restrict__ block_lat,
                                                    const uint32_t total_blocks)
     {
@@ -1376,7 +1382,7 @@ namespace swmi
                 const uint64_t word = t == 0u ? r0 : (t == 1u ? r1 : (t == 2u ? r2 : r3));
                 const uint64_t bits = __ballot(p0 + lane < rows && ((word >> r) & 1ull) != 0ull);
                 if (bits && act != 0u)
-                    MergeEdge<FRESH>(d, bl, bits, va, vb, lb_r, base_r, err, HS[0], SS[0], SK[0], SW[0]);
+                    MergeEdge<FRESH, LOCAL>(d, bl, bits, va, vb, lb_r, base_r, err, HS[0], SS[0], SK[0], SW[0]);
             }
         }
         else
@@ -1392,7 +1398,7 @@ namespace swmi
             Issued(va);
             Issued(vb);
             if (down_bits && act != 0u)
-                MergeEdge<FRESH>(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1], SS[1], SK[1], SW[1]);
+                MergeEdge<FRESH, LOCAL>(d, bl, down_bits, va, vb, lb_b, BlockBase(d, bl.sx, sy_b), err, HS[1], SS[1], SK[1], SW[1]);
         }
     }
 
@@ -2048,13 +2054,26 @@ namespace swmi
 
     // Two-wave merge workgroups resident at once on MI355X: 256 CUs x 16 (8 waves per SIMD).
     constexpr uint32_t kOneRoundMergeBlocks = 4096;
+    // Launches of at least this many blocks unite each edge's pairs in LDS before the lattice-wide unions: many
+    // blocks race on the memory-side atomics there. Smaller launches see few races and only the added latency
+    // (measured at 512 to 2048 blocks: 1024^2 and 2048^2 lattices are faster without).
+    constexpr uint32_t kLocalUnionBlocks = 4096;
+
+    template <bool FRESH>
+    void LaunchMergeAs(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
+    {
+        if (total_blocks >= kLocalUnionBlocks)
+            k_merge<FRESH, true><<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+        else
+            k_merge<FRESH, false><<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+    }
 
     void LaunchMerge(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
     {
         if (total_blocks <= kOneRoundMergeBlocks)
-            k_merge<true><<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            LaunchMergeAs<true>(d_desc, d_block_lat, total_blocks, stream);
         else
-            k_merge<false><<<XcdGrid(total_blocks), 128, 0, stream>>>(d_desc, d_block_lat, total_blocks);
+            LaunchMergeAs<false>(d_desc, d_block_lat, total_blocks, stream);
     }
 
     void LaunchResolve(const LatDesc* d_desc, const uint32_t* d_block_lat, const uint32_t total_blocks, hipStream_t stream)
