@@ -1211,13 +1211,15 @@ namespace swmi
             ca = bl.lb * slots + min(sa, slots - 1u);
             cb = lb_b * slots + min(sb, slots - 1u);
         }
-        // One union per distinct pair of roots (below T_c most bonds of an edge join the same two clusters):
-        // the lanes insert their pairs into the wave's LDS hash set H (128 slots for at most 64 pairs, linear
-        // probing, so every insert ends), and the lane whose insert lands leads the union for its pair.
+        // Without local unions, one lattice-wide union per distinct pair of roots (below T_c most bonds of an edge
+        // join the same two clusters): the lanes insert their pairs into the wave's LDS hash set H (128 slots for
+        // at most 64 pairs, linear probing, so every insert ends), and the lane whose insert lands leads the union
+        // for its pair. With them, every bonded lane takes part in the local unions below, where a repeated pair
+        // costs nothing.
         const uint32_t lo = min(ka, kb), hi = max(ka, kb);
         const bool want = bit && ka != kb;
-        bool leader = false;
-        if (__ballot(want))
+        bool leader = LOCAL && want;
+        if (!LOCAL && __ballot(want))
         {
             H[lane] = 0ull;
             H[64u + lane] = 0ull;
