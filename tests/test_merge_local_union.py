@@ -20,6 +20,7 @@ def _unite(pairs, seed, continue_with_displaced=True):
     key = {x: rng.randrange(1 << 20) * 1000 + i for i, x in enumerate(nodes)}   # distinct keys, as sites are
     entry = {x: (key[x], x) for x in nodes}                                     # a root holds its own key
     lanes = [{"a": a, "b": b, "open": True, "chased": False} for a, b in pairs]
+    linked = []                                                                 # successful links (b under a)
 
     def root(x):
         while entry[x][1] != x:
@@ -47,13 +48,14 @@ def _unite(pairs, seed, continue_with_displaced=True):
         ln["chased"] = False
         if old == eb:
             ln["open"] = False
+            linked.append((b, a))
         elif continue_with_displaced:
             ln["a"], ln["b"] = a, old[1]
         else:
             ln["a"], ln["b"] = a, b
     else:
         raise AssertionError("lanes did not finish")
-    return nodes, key, entry, root
+    return nodes, key, entry, root, linked
 
 
 def _components(pairs, nodes):
@@ -70,7 +72,7 @@ def _components(pairs, nodes):
 
 
 def _check(pairs, seed, continue_with_displaced=True):
-    nodes, key, entry, root = _unite(pairs, seed, continue_with_displaced)
+    nodes, key, entry, root, _ = _unite(pairs, seed, continue_with_displaced)
     find = _components(pairs, nodes)
     for x in nodes:
         if entry[x][1] != x:
@@ -102,3 +104,14 @@ def test_local_union_star_around_one_slot():
 def test_model_detects_a_union_that_drops_the_displaced_parent(count):
     # the retry without the returned parent loses links in some interleaving
     assert any(not _check(pairs, t, continue_with_displaced=False) for t, pairs in _pair_sets(count))
+
+
+def test_successful_local_links_span_the_components():
+    # each successful link joins two different trees, so the links alone connect every component of the pairs
+    # (the alternative of making each lane's own link lattice-wide relies on this)
+    for t, pairs in _pair_sets(1500):
+        nodes, _, _, _, linked = _unite(pairs, t)
+        find_pairs = _components(pairs, nodes)
+        find_links = _components(linked, nodes)
+        assert len(linked) == len(nodes) - len({find_pairs(x) for x in nodes})
+        assert all((find_pairs(a) == find_pairs(b)) == (find_links(a) == find_links(b)) for a in nodes for b in nodes)
