@@ -1254,7 +1254,7 @@ namespace swmi
             return;
         // The pairs of one edge often share a slot (a cluster of one block meets several of its neighbour's), and
         // lanes that link the same root race on one entry of the slot union-find. So the pairs are first united in
-        // LDS, over their distinct slots (H now holds the local entries), and each slot that is not its local
+        // LDS, over their distinct slots (H holds the local entries here), and each slot that is not its local
         // component's root is then linked under that root: one lattice-wide union per lane, no two on one slot.
         S[lane] = 0u;
         S[64u + lane] = 0u;
@@ -1354,7 +1354,7 @@ namespace swmi
         const uint32_t w = Uniform(threadIdx.x >> 6);
         const uint32_t tw = d.tw, rpw = d.rpw, twr = d.twr;
         const gptr<uint32_t> err = Glob(d.state) + kStErr;
-        __shared__ uint64_t HS[2][128];   // per wave: hash set of the root pairs of one edge pass, then the local entries
+        __shared__ uint64_t HS[2][128];   // per wave: the hash set of root pairs (small launches) or the local entries
         __shared__ uint32_t SS[2][128], SK[2][128], SW[2][64];   // per wave: local slots, their keys, the union list
 
         if (w == 0)
