@@ -185,6 +185,29 @@ def test_large_lattice_one_sweep_bit_exact(swmi, oracle, n, T):
         assert e.error_flags() == 0
 
 
+@pytest.mark.parametrize("T", [TC, 2.0])
+def test_large_padded_lattice_one_sweep_bit_exact(swmi, oracle, T):
+    """4100 x 4098: rows padded to a pitch of 4160 and a last block row of 2 lattice rows; 65 x 65 = 4225 blocks,
+    so the merge unites each edge's root pairs in LDS before its lattice-wide unions (launches of at least 4096
+    blocks). One sweep: bonds, labels and spins bit-exact against the oracle; then two more sweeps' spins."""
+    n0, n1 = 4100, 4098
+    print(f"{n0}x{n1} T {T} padded, local unions", flush=True)
+    ref = oracle.PerSite(n0, n1, T, seed=5)
+    with _engine(swmi, n0, n1, T=T, seed=5) as e:
+        assert e.layout()[0] == 4160
+        e.record_bonds(True)
+        e.sweep(1)
+        rb, rl = ref.sweep(record=True)
+        np.testing.assert_array_equal(e.bonds(), rb)
+        np.testing.assert_array_equal(e.labels(), rl)
+        np.testing.assert_array_equal(e.spins(), ref.spins)
+        e.sweep(2)
+        ref.sweep()
+        ref.sweep()
+        np.testing.assert_array_equal(e.spins(), ref.spins)
+        assert e.error_flags() == 0
+
+
 @pytest.mark.parametrize("n0,n1,T,tile_rows", [(64, 64, TC, 0), (128, 96, 2.0, 4), (32, 32, 2.6, 0), (256, 128, TC, 16),
                                                (16, 64, 1.5, 2), (512, 64, 3.0, 64)])
 def test_random_flip_rule_bit_exact(swmi, oracle, n0, n1, T, tile_rows):
